@@ -1,0 +1,311 @@
+#!/usr/bin/env python3
+"""Benchmark of the hcBLAS hot path on one MI355X.
+
+Workload (one "step"): the plain bandwidth set --
+  SGEMV and DGEMV at M = N = 16384, trans N and T, alpha = 1.5, beta = 0.5;
+  SGER and DGER at 8192 x 8192;
+  SAXPY, SSCAL, SCOPY, SDOT, SASUM at n = 2^28 (fp32);
+  DDOT, DASUM at n = 2^27 (fp64);
+all entries uniform[-1,1] (synthetic, seeded, resident in HBM before timing),
+unit stride, called through the C ABI (libhcblas.so).
+
+value = total algorithmic bytes of the step / step time (GB/s); bytes are
+compulsory traffic (each operand read once, outputs written once, y read only
+when beta != 0). The JSON line also carries
+  roofline      -- the dominant kernel (largest share of the step) measured
+                   with HIP events on the stream it is launched on;
+  cpu_baseline  -- the oracle's C restatement timed on the host (1 core) on a
+                   bounded sample of the same workload;
+  ops           -- per-routine GB/s;
+  gemm          -- the GEMM half of the metric: plain SGEMM/DGEMM 8192^3 and
+                   split / Ozaki forms in TFLOP/s with max/mean relative error
+                   against a double-double reference on sampled entries, plus
+                   the 1024^3 row-major SGEMM configuration.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6, "bf16": 2516.6, "int8": 5033.0}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-gemm", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--gemm-n", type=int, default=8192)
+    return ap.parse_args()
+
+
+def bandwidth_ops(hc, h, torch):
+    """(name, algorithmic bytes, callable) for the bandwidth set."""
+    api = hc.api
+    dev = "cuda"
+    g = torch.Generator(device=dev).manual_seed(1234)
+
+    def U(n, dt):
+        return torch.rand(n, generator=g, device=dev, dtype=dt) * 2 - 1
+
+    f32, f64 = torch.float32, torch.float64
+    n_mv = 16384
+    n_r1 = 8192
+    n_s = 1 << 28
+    n_d = 1 << 27
+    A_s = U(n_mv * n_mv, f32)
+    A_d = U(n_mv * n_mv, f64)
+    xs, ys = U(n_mv, f32), U(n_mv, f32)
+    xd, yd = U(n_mv, f64), U(n_mv, f64)
+    G_s = U(n_r1 * n_r1, f32)
+    G_d = U(n_r1 * n_r1, f64)
+    gxs, gys = U(n_r1, f32), U(n_r1, f32)
+    gxd, gyd = U(n_r1, f64), U(n_r1, f64)
+    vx, vy = U(n_s, f32), U(n_s, f32)
+    vz = torch.empty_like(vx)
+    dx, dy = U(n_d, f64), U(n_d, f64)
+
+    def chk(st):
+        if st != 0:
+            raise RuntimeError(f"status {st}")
+
+    mv_s = 4 * (n_mv * n_mv + 3 * n_mv)
+    mv_d = 8 * (n_mv * n_mv + 3 * n_mv)
+    r1_s = 4 * (2 * n_r1 * n_r1 + 2 * n_r1)
+    r1_d = 8 * (2 * n_r1 * n_r1 + 2 * n_r1)
+    ops = [
+        ("sgemv_n", mv_s, lambda: chk(api.hcblasSgemv(h, 0, n_mv, n_mv, 1.5, A_s, n_mv, xs, 1, 0.5, ys, 1))),
+        ("sgemv_t", mv_s, lambda: chk(api.hcblasSgemv(h, 1, n_mv, n_mv, 1.5, A_s, n_mv, xs, 1, 0.5, ys, 1))),
+        ("dgemv_n", mv_d, lambda: chk(api.hcblasDgemv(h, 0, n_mv, n_mv, 1.5, A_d, n_mv, xd, 1, 0.5, yd, 1))),
+        ("dgemv_t", mv_d, lambda: chk(api.hcblasDgemv(h, 1, n_mv, n_mv, 1.5, A_d, n_mv, xd, 1, 0.5, yd, 1))),
+        ("sger", r1_s, lambda: chk(api.hcblasSger(h, n_r1, n_r1, 1.5, gxs, 1, gys, 1, G_s, n_r1))),
+        ("dger", r1_d, lambda: chk(api.hcblasDger(h, n_r1, n_r1, 1.5, gxd, 1, gyd, 1, G_d, n_r1))),
+        ("saxpy", 3 * 4 * n_s, lambda: chk(api.hcblasSaxpy(h, n_s, 1.5, vx, 1, vy, 1))),
+        ("sscal", 2 * 4 * n_s, lambda: chk(api.hcblasSscal(h, n_s, 1.0, vy, 1))),
+        ("scopy", 2 * 4 * n_s, lambda: chk(api.hcblasScopy(h, n_s, vx, 1, vz, 1))),
+        ("sdot", 2 * 4 * n_s, lambda: chk(api.hcblasSdot(h, n_s, vx, 1, vy, 1, None)[0])),
+        ("sasum", 4 * n_s, lambda: chk(api.hcblasSasum(h, n_s, vx, 1, None)[0])),
+        ("ddot", 2 * 8 * n_d, lambda: chk(api.hcblasDdot(h, n_d, dx, 1, dy, 1, None)[0])),
+        ("dasum", 8 * n_d, lambda: chk(api.hcblasDasum(h, n_d, dx, 1, None)[0])),
+    ]
+    return ops
+
+
+def time_bandwidth(torch, ops, steps, warmup):
+    for _ in range(warmup):
+        for _, _, fn in ops:
+            fn()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True),
+           torch.cuda.Event(enable_timing=True)) for _ in ops]
+    per_op = [0.0] * len(ops)
+    t0 = time.perf_counter()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(steps):
+        for i, (_, _, fn) in enumerate(ops):
+            ev[i][0].record()
+            fn()
+            ev[i][1].record()
+        torch.cuda.synchronize()
+        for i in range(len(ops)):
+            per_op[i] += ev[i][0].elapsed_time(ev[i][1])
+    end.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    step_ms = start.elapsed_time(end) / steps
+    per_op = [t / steps for t in per_op]
+    return step_ms, per_op, wall
+
+
+def gemm_section(hc, h, torch, oracle, n, reps=5):
+    """Plain vs split GEMM at n^3 (NN, column-major, uniform[-1,1])."""
+    import numpy as np
+    api = hc.api
+    out = {}
+    g = torch.Generator(device="cuda").manual_seed(7)
+    rng = np.random.default_rng(0)
+    ns = 256
+    rows = rng.integers(0, n, ns)
+    cols = rng.integers(0, n, ns)
+
+    def run(name, dt, fn, A, B, C, peak_key):
+        st = fn()
+        if st != 0:
+            out[name] = {"status": int(st)}
+            return
+        torch.cuda.synchronize()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        times = []
+        for _ in range(reps):
+            s.record()
+            fn()
+            e.record()
+            torch.cuda.synchronize()
+            times.append(s.elapsed_time(e))
+        ms = sorted(times)[len(times) // 2]
+        tflops = 2.0 * n * n * n / (ms * 1e-3) / 1e12
+        # error vs double-double on sampled entries (C = A*B, col-major)
+        Ah = A.view(n, n)  # Ah[k, m] = A(m, k)
+        Bh = B.view(n, n)
+        Ch = C.view(n, n)
+        a_rows = Ah[:, torch.as_tensor(rows, device="cuda")].t().contiguous().cpu().numpy()
+        b_cols = Bh[torch.as_tensor(cols, device="cuda"), :].contiguous().cpu().numpy()
+        got = Ch[torch.as_tensor(cols, device="cuda"), torch.as_tensor(rows, device="cuda")].cpu().numpy()
+        ref = np.empty(ns)
+        for i in range(ns):
+            ref[i] = oracle.gemm_sample_exact(1, 0, 0, 1, 1, n, 1.0, a_rows[i].copy(), 1,
+                                              b_cols[i].copy(), n, 0.0,
+                                              np.zeros(1, a_rows.dtype), 1,
+                                              np.array([0]), np.array([0]))[0]
+        rel = np.abs(got.astype(np.float64) - ref) / np.maximum(np.abs(ref), 1e-300)
+        out[name] = {"tflops": round(tflops, 2), "ms": round(ms, 3),
+                     "peak_frac": round(tflops / PEAK_TFLOPS[peak_key], 4),
+                     "max_rel_err": float(rel.max()), "mean_rel_err": float(rel.mean())}
+
+    for dt, tag, pk in ((torch.float32, "s", "fp32"), (torch.float64, "d", "fp64")):
+        A = torch.rand(n * n, generator=g, device="cuda", dtype=dt) * 2 - 1
+        B = torch.rand(n * n, generator=g, device="cuda", dtype=dt) * 2 - 1
+        C = torch.zeros(n * n, device="cuda", dtype=dt)
+        fn = getattr(api, f"hcblas{tag.upper()}gemm")
+        run(f"{tag}gemm_plain", dt, lambda: fn(h, 0, 0, n, n, n, 1.0, A, n, B, n, 0.0, C, n),
+            A, B, C, pk)
+        if tag == "s":
+            split = api.hcblas_sgemm_split_bf16
+            run("sgemm_split_bf16", dt,
+                lambda: split(h, 1, 0, 0, n, n, n, 1.0, A, n, B, n, 0.0, C, n, 0, 0, 0),
+                A, B, C, pk)
+        else:
+            oz = api.hcblas_dgemm_ozaki_int8
+            run("dgemm_ozaki_int8", dt,
+                lambda: oz(h, 1, 0, 0, n, n, n, 1.0, A, n, B, n, 0.0, C, n, 0, 0, 0),
+                A, B, C, pk)
+        del A, B, C
+        torch.cuda.empty_cache()
+    # 1024^3 row-major NN SGEMM (plain-SGEMM configuration)
+    m = 1024
+    A = torch.rand(m * m, generator=g, device="cuda") * 2 - 1
+    B = torch.rand(m * m, generator=g, device="cuda") * 2 - 1
+    C = torch.zeros(m * m, device="cuda")
+    fn = lambda: api.hcblas_sgemm(h, 0, 0, 0, m, m, m, 1.0, A, m, B, m, 0.0, C, m, 0, 0, 0)
+    fn()
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ts = []
+    for _ in range(20):
+        s.record()
+        fn()
+        e.record()
+        torch.cuda.synchronize()
+        ts.append(s.elapsed_time(e))
+    ms = sorted(ts)[len(ts) // 2]
+    out["sgemm_rowmajor_1024"] = {"tflops": round(2.0 * m ** 3 / (ms * 1e-3) / 1e12, 2),
+                                  "ms": round(ms, 4)}
+    return out
+
+
+def cpu_baseline(oracle):
+    """The oracle's C restatement (single thread) on a bounded sample of the
+    bandwidth set: gemv/ger at 8192^2, level 1 at n = 2^25 / 2^24."""
+    import numpy as np
+    n_mv, n_s, n_d = 8192, 1 << 25, 1 << 24
+    A_s = oracle.uniform_f32(n_mv * n_mv, 1)
+    A_d = oracle.uniform_f64(n_mv * n_mv, 2)
+    xs, ys = oracle.uniform_f32(n_mv, 3), oracle.uniform_f32(n_mv, 4)
+    xd, yd = oracle.uniform_f64(n_mv, 5), oracle.uniform_f64(n_mv, 6)
+    vx, vy = oracle.uniform_f32(n_s, 7), oracle.uniform_f32(n_s, 8)
+    dx, dy = oracle.uniform_f64(n_d, 9), oracle.uniform_f64(n_d, 10)
+    L = oracle.lib()
+    P = lambda a: a.ctypes.data
+    mv_s = 4 * (n_mv * n_mv + 3 * n_mv)
+    mv_d = 8 * (n_mv * n_mv + 3 * n_mv)
+    r1_s = 4 * (2 * n_mv * n_mv + 2 * n_mv)
+    r1_d = 8 * (2 * n_mv * n_mv + 2 * n_mv)
+    jobs = [
+        (mv_s, lambda: L.orc_sgemv(1, 0, n_mv, n_mv, 1.5, P(A_s), n_mv, P(xs), 1, 0.5, P(ys), 1)),
+        (mv_s, lambda: L.orc_sgemv(1, 1, n_mv, n_mv, 1.5, P(A_s), n_mv, P(xs), 1, 0.5, P(ys), 1)),
+        (mv_d, lambda: L.orc_dgemv(1, 0, n_mv, n_mv, 1.5, P(A_d), n_mv, P(xd), 1, 0.5, P(yd), 1)),
+        (mv_d, lambda: L.orc_dgemv(1, 1, n_mv, n_mv, 1.5, P(A_d), n_mv, P(xd), 1, 0.5, P(yd), 1)),
+        (r1_s, lambda: L.orc_sger(1, n_mv, n_mv, 1.5, P(xs), 1, P(ys), 1, P(A_s), n_mv)),
+        (r1_d, lambda: L.orc_dger(1, n_mv, n_mv, 1.5, P(xd), 1, P(yd), 1, P(A_d), n_mv)),
+        (12 * n_s, lambda: L.orc_saxpy(n_s, 1.5, P(vx), 1, P(vy), 1)),
+        (8 * n_s, lambda: L.orc_sscal(n_s, 1.0, P(vy), 1)),
+        (8 * n_s, lambda: L.orc_scopy(n_s, P(vx), 1, P(vy), 1)),
+        (8 * n_s, lambda: L.orc_sdot(n_s, P(vx), 1, P(vy), 1)),
+        (4 * n_s, lambda: L.orc_sasum(n_s, P(vx), 1)),
+        (16 * n_d, lambda: L.orc_ddot(n_d, P(dx), 1, P(dy), 1)),
+        (8 * n_d, lambda: L.orc_dasum(n_d, P(dx), 1)),
+    ]
+    total_b, total_t = 0, 0.0
+    for reps in range(2):
+        for b, fn in jobs:
+            t0 = time.perf_counter()
+            fn()
+            total_t += time.perf_counter() - t0
+            total_b += b
+    del np
+    return {"value": round(total_b / total_t / 1e9, 3), "unit": "GB/s", "cores": 1,
+            "kind": "port",
+            "sample": "bandwidth set at reduced size, 2 passes: S/DGEMV N+T and "
+                      "S/DGER at 8192^2, SAXPY/SSCAL/SCOPY/SDOT/SASUM n=2^25, "
+                      "DDOT/DASUM n=2^24 (oracle/refblas.c, netlib loop order)",
+            "seconds": round(total_t, 2)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+    import torch
+    import hcblas_amd as hc
+
+    torch.cuda.set_device(0)
+    h = hc.Handle()
+    h.set_stream(torch.cuda.current_stream())
+    ops = bandwidth_ops(hc, h, torch)
+    step_ms, per_op, wall = time_bandwidth(torch, ops, args.steps, args.warmup)
+    total_bytes = sum(b for _, b, _ in ops)
+    value = total_bytes / (step_ms * 1e-3) / 1e9
+    op_gbs = {name: round(b / (t * 1e-3) / 1e9, 1) for (name, b, _), t in zip(ops, per_op)}
+    dom = max(range(len(ops)), key=lambda i: per_op[i])
+    dname, dbytes, _ = ops[dom]
+    achieved = dbytes / (per_op[dom] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": dname, "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "avg_ms": round(per_op[dom], 4)}
+    result = {
+        "metric": "GEMM TFLOP/s split vs plain at equal err vs FP64; GEMV/dot HBM GB/s",
+        "value": round(value, 1), "unit": "GB/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(step_ms, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32+fp64",
+        "data": "synthetic uniform[-1,1], seeded, resident in HBM",
+        "config": {"workload": "plain bandwidth set: S/DGEMV 16384^2 N+T a=1.5 b=0.5, "
+                               "S/DGER 8192^2, SAXPY/SSCAL/SCOPY/SDOT/SASUM n=2^28, "
+                               "DDOT/DASUM n=2^27", "bytes_per_step": total_bytes},
+        "roofline": roofline, "ops": op_gbs,
+    }
+    if not args.no_gemm:
+        import oracle
+        del ops
+        torch.cuda.empty_cache()
+        result["gemm"] = gemm_section(hc, h, torch, oracle, args.gemm_n)
+    if not args.no_cpu:
+        import oracle
+        result["cpu_baseline"] = cpu_baseline(oracle)
+    h.destroy()
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

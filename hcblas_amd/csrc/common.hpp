@@ -1,0 +1,86 @@
+// Internal definitions shared by every translation unit of libhcblas.
+//
+// The handle plays the role of the reference's Hcblaslibrary context
+// (lib/include/hcblaslib.h:67-101): it owns the device, the stream, and the
+// persistent device workspaces that the reference allocated and freed on every
+// call (sdot_array.cpp:40-41, sgemv_array.cpp:41; here allocated once).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstddef>
+#include <cstdint>
+
+#include "../../include/hcblas.h"
+
+namespace hcb {
+
+constexpr uint32_t kHandleMagic = 0x48434d49u;  // "HCMI"
+
+// Reduction scratch: one double per workgroup partial, plus the final value.
+constexpr int kMaxPartials = 8192;
+
+struct Workspace {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace hcb
+
+struct hcblasContext {
+  uint32_t magic = 0;
+  int device = 0;
+  int num_cus = 256;
+  hipStream_t stream = nullptr;
+  int math_mode = HCBLAS_DEFAULT_MATH;
+  int ozaki_slices = 7;
+
+  // dot / asum: per-workgroup partials, a ticket counter that the last
+  // workgroup resets to zero, the device result and a pinned host mirror.
+  double *d_partials = nullptr;
+  unsigned int *d_ticket = nullptr;
+  double *d_result = nullptr;
+  double *h_result = nullptr;
+
+  // gemv split-reduction partials and per-row-block tickets (grow-only; the
+  // ticket region is zeroed when it is (re)allocated and every launch leaves
+  // it zero again).
+  hcb::Workspace gemv_partials;
+  hcb::Workspace gemv_tickets;
+
+  // Packing / slicing buffers of the complex, split and Ozaki GEMM paths.
+  hcb::Workspace gemm_ws;
+};
+
+namespace hcb {
+
+inline bool handle_ok(hcblasHandle_t h) {
+  return h != nullptr && h->magic == kHandleMagic;
+}
+
+// Makes `ws` at least `bytes` large. Growing frees the old block, which
+// synchronises the device first (hipFree semantics), so work still in flight
+// on the old block is complete before it is released.
+hipError_t ensure_workspace(Workspace &ws, size_t bytes);
+
+// Makes the gemv ticket region at least `count` counters and zero.
+hipError_t ensure_tickets(hcblasHandle_t h, size_t count);
+
+inline hcblasStatus_t launch_status() {
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? HCBLAS_STATUS_SUCCESS
+                         : HCBLAS_STATUS_EXECUTION_FAILED;
+}
+
+inline bool aligned16(const void *p) {
+  return (reinterpret_cast<uintptr_t>(p) & 15u) == 0;
+}
+
+inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+__host__ __device__ inline int64_t cdiv_dev(int64_t a, int64_t b) {
+  return (a + b - 1) / b;
+}
+
+}  // namespace hcb

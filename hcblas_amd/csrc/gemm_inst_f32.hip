@@ -1,0 +1,8 @@
+// Template instances of the MFMA GEMM for one element type.
+#include "gemm_launch_impl.hpp"
+
+namespace hcb {
+template hcblasStatus_t launch_gemm<Elem::F32, float>(hipStream_t, int,
+                                                     GemmParams, bool, bool,
+                                                     bool, int64_t);
+}  // namespace hcb

@@ -1,0 +1,439 @@
+// Generic MFMA GEMM macro-tile kernel for gfx950.
+//
+// Replaces the VALU register-micro-tile kernels of the reference
+// (e.g. sgemm_NN_kernels.cpp:1360-1469 MICRO_TS16XMTS2, :137
+// Mini_Batch_M128_N128_K16_TS16XMTS4_MB2; dgemm/hgemm copies of the same).
+//
+// Problem (column-major storage, after the host has folded row-major order
+// into an operand swap):  C[M x N] = alpha * A'[M x K] * B'[K x N] + beta * C
+// where A' element (m,k) is A[k + m*lda] when A_K (k contiguous) and
+// A[m + k*lda] otherwise; B' element (k,n) is B[k + n*ldb] when B_K and
+// B[n + k*ldb] otherwise.
+//
+// Tiling:
+//   - workgroup = 256 threads = 4 waves in a WM x WN grid, macro tile BM x BN,
+//     K step BK; each wave owns a (BM/WM) x (BN/WN) sub-tile made of 16x16
+//     MFMA blocks (v_mfma_*_16x16x{4,4,32,32,64}).
+//   - LDS holds both operands k-contiguous ([rows][BK] + 16 bytes of padding
+//     per row, which makes the 16-byte fragment reads of 16 rows
+//     bank-conflict-free), double-buffered: global loads for tile k+1 are in
+//     flight in registers while the MFMAs consume tile k, and one barrier per
+//     K step separates the two buffers.
+//   - a lane reads one 16-byte fragment per operand per chunk; the four lane
+//     groups (lane/16) hold four disjoint k sub-ranges, and the same mapping is
+//     used for A and B, so the MFMA sums exactly the products of the chunk.
+//   - MFMA output layout (16x16, four values per lane): value r of lane l is
+//     C(4*(l/16) + r, l%16).
+//   - tile order: consecutive workgroup ids are spread over the 8 XCDs by the
+//     dispatcher, so ids are remapped to give each XCD a contiguous slice of
+//     a GROUP_M-grouped tile order (A and B panels stay in that XCD's L2).
+#pragma once
+
+#include "common.hpp"
+
+namespace hcb {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+typedef int intx4 __attribute__((ext_vector_type(4)));
+typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+enum class Elem { F32, F64, F16, BF16, I8 };
+
+template <Elem E>
+struct ElemT;
+template <>
+struct ElemT<Elem::F32> {
+  using T = float;
+  using Acc = float;
+  using AccV = floatx4;
+  static constexpr int VEC = 4;    // elements per 16-byte fragment
+  static constexpr int STEPS = 4;  // MFMAs per fragment pair
+};
+template <>
+struct ElemT<Elem::F64> {
+  using T = double;
+  using Acc = double;
+  using AccV = doublex4;
+  static constexpr int VEC = 2;
+  static constexpr int STEPS = 2;
+};
+template <>
+struct ElemT<Elem::F16> {
+  using T = uint16_t;  // binary16 bits
+  using Acc = float;
+  using AccV = floatx4;
+  static constexpr int VEC = 8;
+  static constexpr int STEPS = 1;
+};
+template <>
+struct ElemT<Elem::BF16> {
+  using T = uint16_t;  // bfloat16 bits
+  using Acc = float;
+  using AccV = floatx4;
+  static constexpr int VEC = 8;
+  static constexpr int STEPS = 1;
+};
+template <>
+struct ElemT<Elem::I8> {
+  using T = int8_t;
+  using Acc = int;
+  using AccV = intx4;
+  static constexpr int VEC = 16;
+  static constexpr int STEPS = 1;
+};
+
+template <typename To, typename From>
+__device__ __forceinline__ To bitcast(const From &f) {
+  static_assert(sizeof(To) == sizeof(From), "bitcast size");
+  To t;
+  __builtin_memcpy(&t, &f, sizeof(To));
+  return t;
+}
+
+// One MFMA step s of a fragment pair.
+template <Elem E, int S>
+__device__ __forceinline__ void mfma_step(const uint4 &a, const uint4 &b,
+                                          typename ElemT<E>::AccV &c);
+
+template <>
+__device__ __forceinline__ void mfma_step<Elem::F32, 0>(const uint4 &a,
+                                                        const uint4 &b,
+                                                        floatx4 &c) {
+  c = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(a.x),
+                                           __uint_as_float(b.x), c, 0, 0, 0);
+}
+template <>
+__device__ __forceinline__ void mfma_step<Elem::F32, 1>(const uint4 &a,
+                                                        const uint4 &b,
+                                                        floatx4 &c) {
+  c = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(a.y),
+                                           __uint_as_float(b.y), c, 0, 0, 0);
+}
+template <>
+__device__ __forceinline__ void mfma_step<Elem::F32, 2>(const uint4 &a,
+                                                        const uint4 &b,
+                                                        floatx4 &c) {
+  c = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(a.z),
+                                           __uint_as_float(b.z), c, 0, 0, 0);
+}
+template <>
+__device__ __forceinline__ void mfma_step<Elem::F32, 3>(const uint4 &a,
+                                                        const uint4 &b,
+                                                        floatx4 &c) {
+  c = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(a.w),
+                                           __uint_as_float(b.w), c, 0, 0, 0);
+}
+template <>
+__device__ __forceinline__ void mfma_step<Elem::F64, 0>(const uint4 &a,
+                                                        const uint4 &b,
+                                                        doublex4 &c) {
+  double av = bitcast<double>(make_uint2(a.x, a.y));
+  double bv = bitcast<double>(make_uint2(b.x, b.y));
+  c = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, c, 0, 0, 0);
+}
+template <>
+__device__ __forceinline__ void mfma_step<Elem::F64, 1>(const uint4 &a,
+                                                        const uint4 &b,
+                                                        doublex4 &c) {
+  double av = bitcast<double>(make_uint2(a.z, a.w));
+  double bv = bitcast<double>(make_uint2(b.z, b.w));
+  c = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, c, 0, 0, 0);
+}
+template <>
+__device__ __forceinline__ void mfma_step<Elem::F16, 0>(const uint4 &a,
+                                                        const uint4 &b,
+                                                        floatx4 &c) {
+  c = __builtin_amdgcn_mfma_f32_16x16x32_f16(bitcast<halfx8>(a),
+                                             bitcast<halfx8>(b), c, 0, 0, 0);
+}
+template <>
+__device__ __forceinline__ void mfma_step<Elem::BF16, 0>(const uint4 &a,
+                                                         const uint4 &b,
+                                                         floatx4 &c) {
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bitcast<bf16x8>(a),
+                                              bitcast<bf16x8>(b), c, 0, 0, 0);
+}
+template <>
+__device__ __forceinline__ void mfma_step<Elem::I8, 0>(const uint4 &a,
+                                                       const uint4 &b,
+                                                       intx4 &c) {
+  c = __builtin_amdgcn_mfma_i32_16x16x64_i8(bitcast<intx4>(a),
+                                            bitcast<intx4>(b), c, 0, 0, 0);
+}
+
+// ---------------------------------------------------------------------------
+// Global -> register -> LDS tile loader for one operand.
+// ROWS = tile extent in m (or n); the LDS tile is [ROWS][BK] k-contiguous.
+// KCONT: the global operand is k-contiguous (element (r,k) at P[k + r*ld]).
+// VLOAD: 16-byte global loads are legal (base, ld aligned to the vector).
+// ---------------------------------------------------------------------------
+template <typename T, int ROWS, int BK, bool KCONT, bool VLOAD>
+struct TileLoader {
+  static constexpr int VEC = 16 / (int)sizeof(T);
+  static constexpr int NV = ROWS * BK / VEC;  // 16-byte vectors per tile
+  static constexpr int PER_THREAD = (NV + 255) / 256;
+  static constexpr int ROWB = BK * (int)sizeof(T) + 16;  // LDS row stride
+  uint4 reg[PER_THREAD];
+
+  __device__ __forceinline__ void load(const T *__restrict__ P, int64_t ld,
+                                       int r0, int k0, int R, int K) {
+#pragma unroll
+    for (int i = 0; i < PER_THREAD; ++i) {
+      const int v = threadIdx.x + i * 256;
+      if (NV % 256 != 0 && v >= NV) continue;
+      int r, k;
+      if (KCONT) {
+        r = v / (BK / VEC);
+        k = (v % (BK / VEC)) * VEC;
+      } else {
+        k = v / (ROWS / VEC);
+        r = (v % (ROWS / VEC)) * VEC;
+      }
+      const int gr = r0 + r, gk = k0 + k;
+      if (KCONT) {
+        const T *src = P + (int64_t)gr * ld + gk;
+        if (VLOAD && gr < R && gk + VEC <= K) {
+          reg[i] = *reinterpret_cast<const uint4 *>(src);
+        } else {
+          T t[VEC];
+#pragma unroll
+          for (int e = 0; e < VEC; ++e)
+            t[e] = (gr < R && gk + e < K) ? src[e] : T(0);
+          __builtin_memcpy(&reg[i], t, 16);
+        }
+      } else {
+        const T *src = P + (int64_t)gk * ld + gr;
+        if (VLOAD && gk < K && gr + VEC <= R) {
+          reg[i] = *reinterpret_cast<const uint4 *>(src);
+        } else {
+          T t[VEC];
+#pragma unroll
+          for (int e = 0; e < VEC; ++e)
+            t[e] = (gk < K && gr + e < R) ? src[e] : T(0);
+          __builtin_memcpy(&reg[i], t, 16);
+        }
+      }
+    }
+  }
+
+  __device__ __forceinline__ void store(char *lds) const {
+#pragma unroll
+    for (int i = 0; i < PER_THREAD; ++i) {
+      const int v = threadIdx.x + i * 256;
+      if (NV % 256 != 0 && v >= NV) continue;
+      if (KCONT) {
+        const int r = v / (BK / VEC);
+        const int k = (v % (BK / VEC)) * VEC;
+        *reinterpret_cast<uint4 *>(lds + r * ROWB + k * (int)sizeof(T)) =
+            reg[i];
+      } else {
+        const int k = v / (ROWS / VEC);
+        const int r = (v % (ROWS / VEC)) * VEC;
+        T t[VEC];
+        __builtin_memcpy(t, &reg[i], 16);
+#pragma unroll
+        for (int e = 0; e < VEC; ++e)
+          *reinterpret_cast<T *>(lds + (r + e) * ROWB + k * (int)sizeof(T)) =
+              t[e];
+      }
+    }
+  }
+};
+
+// Tile id -> (tile_m, tile_n): XCD-contiguous slices of a GROUP_M-grouped order.
+__device__ __forceinline__ void tile_coords(int pid, int tm, int tn,
+                                            int &tile_m, int &tile_n) {
+  constexpr int kXcd = 8;
+  constexpr int GROUP_M = 8;
+  const int nt = tm * tn;
+  if (nt % kXcd == 0) pid = (pid % kXcd) * (nt / kXcd) + pid / kXcd;
+  const int width = GROUP_M * tn;
+  const int group = pid / width;
+  const int first_m = group * GROUP_M;
+  const int gsize = (tm - first_m) < GROUP_M ? (tm - first_m) : GROUP_M;
+  const int in_group = pid % width;
+  tile_m = first_m + in_group % gsize;
+  tile_n = in_group / gsize;
+}
+
+// Output conversion.
+template <typename O, typename Acc>
+__device__ __forceinline__ O to_out(Acc v) {
+  return (O)v;
+}
+template <>
+__device__ __forceinline__ uint16_t to_out<uint16_t, float>(float v) {
+  return bitcast<uint16_t>((_Float16)v);
+}
+template <typename Acc, typename O>
+__device__ __forceinline__ Acc from_out(O v) {
+  return (Acc)v;
+}
+template <>
+__device__ __forceinline__ float from_out<float, uint16_t>(uint16_t v) {
+  return (float)bitcast<_Float16>(v);
+}
+
+__device__ __forceinline__ float fma_acc(float a, float b, float c) {
+  return __builtin_fmaf(a, b, c);
+}
+__device__ __forceinline__ double fma_acc(double a, double b, double c) {
+  return __builtin_fma(a, b, c);
+}
+
+struct GemmParams {
+  int M, N, K;
+  const void *A;
+  int64_t lda, strideA;
+  const void *B;
+  int64_t ldb, strideB;
+  void *C;
+  int64_t ldc, strideC;
+  // Pointer-array batches (device arrays); nullptr selects the strided form.
+  const void *const *Aptrs;
+  const void *const *Bptrs;
+  void *const *Cptrs;
+  int64_t aOff, bOff, cOff;
+  double alpha, beta;
+  int tiles_m, tiles_n;
+};
+
+template <Elem E, typename OutT, bool A_K, bool B_K, int BM, int BN, int BK,
+          int WM, int WN, bool VA, bool VB>
+__global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmParams p) {
+  using Tr = ElemT<E>;
+  using T = typename Tr::T;
+  using Acc = typename Tr::Acc;
+  using AccV = typename Tr::AccV;
+  constexpr int VEC = Tr::VEC;
+  constexpr int CK = 4 * VEC;  // k per fragment chunk
+  static_assert(BK % CK == 0, "BK must be a multiple of the chunk");
+  static_assert(WM * WN == 4, "four waves");
+  constexpr int WTM = BM / WM, WTN = BN / WN;
+  constexpr int FM = WTM / 16, FN = WTN / 16;
+  static_assert(FM * 16 == WTM && FN * 16 == WTN, "wave tile");
+
+  using LA = TileLoader<T, BM, BK, A_K, VA>;
+  using LB = TileLoader<T, BN, BK, B_K, VB>;
+  constexpr int ROWB = LA::ROWB;
+  constexpr int A_BYTES = BM * ROWB, B_BYTES = BN * ROWB;
+  __shared__ __attribute__((aligned(16))) char lds[2 * (A_BYTES + B_BYTES)];
+
+  const int z = blockIdx.z;
+  const T *A;
+  const T *B;
+  OutT *C;
+  if (p.Aptrs)
+    A = static_cast<const T *>(p.Aptrs[z]) + p.aOff;
+  else
+    A = static_cast<const T *>(p.A) + p.aOff + (int64_t)z * p.strideA;
+  if (p.Bptrs)
+    B = static_cast<const T *>(p.Bptrs[z]) + p.bOff;
+  else
+    B = static_cast<const T *>(p.B) + p.bOff + (int64_t)z * p.strideB;
+  if (p.Cptrs)
+    C = static_cast<OutT *>(p.Cptrs[z]) + p.cOff;
+  else
+    C = static_cast<OutT *>(p.C) + p.cOff + (int64_t)z * p.strideC;
+
+  int tile_m, tile_n;
+  tile_coords(blockIdx.x, p.tiles_m, p.tiles_n, tile_m, tile_n);
+  const int m0 = tile_m * BM, n0 = tile_n * BN;
+
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wm = wave / WN, wn = wave % WN;
+  const int lrow = lane & 15, lgrp = lane >> 4;
+
+  AccV acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = AccV{};
+
+  LA la;
+  LB lb;
+  const int nk = (p.K + BK - 1) / BK;
+  la.load(A, p.lda, m0, 0, p.M, p.K);
+  lb.load(B, p.ldb, n0, 0, p.N, p.K);
+  la.store(lds);
+  lb.store(lds + A_BYTES);
+  __syncthreads();
+
+  for (int kt = 0; kt < nk; ++kt) {
+    char *cur = lds + (kt & 1) * (A_BYTES + B_BYTES);
+    char *nxt = lds + ((kt + 1) & 1) * (A_BYTES + B_BYTES);
+    const bool more = kt + 1 < nk;
+    if (more) {
+      la.load(A, p.lda, m0, (kt + 1) * BK, p.M, p.K);
+      lb.load(B, p.ldb, n0, (kt + 1) * BK, p.N, p.K);
+    }
+    const char *As = cur;
+    const char *Bs = cur + A_BYTES;
+#pragma unroll
+    for (int c = 0; c < BK / CK; ++c) {
+      uint4 af[FM], bf[FN];
+      const int kbyte = (c * CK + lgrp * VEC) * (int)sizeof(T);
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+        af[i] = *reinterpret_cast<const uint4 *>(
+            As + (wm * WTM + i * 16 + lrow) * ROWB + kbyte);
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+        bf[j] = *reinterpret_cast<const uint4 *>(
+            Bs + (wn * WTN + j * 16 + lrow) * ROWB + kbyte);
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) mfma_step<E, 0>(af[i], bf[j], acc[i][j]);
+      if constexpr (Tr::STEPS > 1) {
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j)
+            mfma_step<E, 1>(af[i], bf[j], acc[i][j]);
+      }
+      if constexpr (Tr::STEPS > 2) {
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j)
+            mfma_step<E, 2>(af[i], bf[j], acc[i][j]);
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j)
+            mfma_step<E, 3>(af[i], bf[j], acc[i][j]);
+      }
+    }
+    if (more) {
+      la.store(nxt);
+      lb.store(nxt + A_BYTES);
+    }
+    __syncthreads();
+  }
+
+  // Epilogue: C = alpha*acc + beta*C; beta == 0 never reads C.
+  const Acc alpha = (Acc)p.alpha, beta = (Acc)p.beta;
+#pragma unroll
+  for (int i = 0; i < FM; ++i) {
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int col = n0 + wn * WTN + j * 16 + lrow;
+      if (col >= p.N) continue;
+      const int rowb = m0 + wm * WTM + i * 16 + lgrp * 4;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = rowb + r;
+        if (row >= p.M) continue;
+        OutT *dst = C + (int64_t)col * p.ldc + row;
+        Acc v = alpha * (Acc)acc[i][j][r];
+        if (beta != Acc(0)) v = fma_acc(beta, from_out<Acc>(*dst), v);
+        *dst = to_out<OutT>(v);
+      }
+    }
+  }
+}
+
+}  // namespace hcb

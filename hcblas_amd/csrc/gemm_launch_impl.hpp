@@ -1,0 +1,77 @@
+// Definition of launch_gemm; included once per element type so the template
+// instances compile in parallel translation units.
+#pragma once
+
+#include "gemm_launch.hpp"
+
+namespace hcb {
+
+template <Elem E>
+struct TileCfg;
+template <>
+struct TileCfg<Elem::F32> {
+  static constexpr int BK_BIG = 32, BK_SMALL = 32;
+};
+template <>
+struct TileCfg<Elem::F64> {
+  static constexpr int BK_BIG = 16, BK_SMALL = 16;
+};
+template <>
+struct TileCfg<Elem::F16> {
+  static constexpr int BK_BIG = 64, BK_SMALL = 64;
+};
+template <>
+struct TileCfg<Elem::BF16> {
+  static constexpr int BK_BIG = 64, BK_SMALL = 64;
+};
+
+template <Elem E, typename OutT, int BM, int BN, int BK, int WM, int WN>
+hcblasStatus_t launch_tiles(hipStream_t stream, GemmParams p, bool a_k,
+                            bool b_k, bool vload, int64_t batch) {
+  p.tiles_m = (int)cdiv(p.M, BM);
+  p.tiles_n = (int)cdiv(p.N, BN);
+  const unsigned tiles = (unsigned)(p.tiles_m * p.tiles_n);
+  for (int64_t b0 = 0; b0 < batch; b0 += 65535) {
+    GemmParams q = p;
+    const int64_t nb = batch - b0 < 65535 ? batch - b0 : 65535;
+    if (b0) {
+      // Advance strided bases / pointer arrays to this chunk of the batch.
+      if (q.Aptrs) q.Aptrs += b0;
+      else q.A = (const char *)q.A + b0 * q.strideA * sizeof(typename ElemT<E>::T);
+      if (q.Bptrs) q.Bptrs += b0;
+      else q.B = (const char *)q.B + b0 * q.strideB * sizeof(typename ElemT<E>::T);
+      if (q.Cptrs) q.Cptrs += b0;
+      else q.C = (char *)q.C + b0 * q.strideC * sizeof(OutT);
+    }
+    dim3 grid(tiles, 1, (unsigned)nb);
+#define HCB_GEMM_CASE(AK, BKC, VL)                                        \
+  if (a_k == AK && b_k == BKC && vload == VL)                             \
+    gemm_mfma_kernel<E, OutT, AK, BKC, BM, BN, BK, WM, WN, VL, VL>        \
+        <<<grid, 256, 0, stream>>>(q);
+    HCB_GEMM_CASE(false, false, false)
+    HCB_GEMM_CASE(false, true, false)
+    HCB_GEMM_CASE(true, false, false)
+    HCB_GEMM_CASE(true, true, false)
+    HCB_GEMM_CASE(false, false, true)
+    HCB_GEMM_CASE(false, true, true)
+    HCB_GEMM_CASE(true, false, true)
+    HCB_GEMM_CASE(true, true, true)
+#undef HCB_GEMM_CASE
+    hcblasStatus_t st = launch_status();
+    if (st != HCBLAS_STATUS_SUCCESS) return st;
+  }
+  return HCBLAS_STATUS_SUCCESS;
+}
+
+template <Elem E, typename OutT>
+hcblasStatus_t launch_gemm(hipStream_t stream, int num_cus, GemmParams p,
+                           bool a_k, bool b_k, bool vload, int64_t batch) {
+  const int64_t big_tiles = cdiv(p.M, 128) * cdiv(p.N, 128) * batch;
+  if (big_tiles >= num_cus)
+    return launch_tiles<E, OutT, 128, 128, TileCfg<E>::BK_BIG, 2, 2>(
+        stream, p, a_k, b_k, vload, batch);
+  return launch_tiles<E, OutT, 64, 64, TileCfg<E>::BK_SMALL, 2, 2>(
+      stream, p, a_k, b_k, vload, batch);
+}
+
+}  // namespace hcb

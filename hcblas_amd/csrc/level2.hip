@@ -1,0 +1,633 @@
+// Level-2 BLAS on gfx950: {S,D}gemv and {S,D}ger, plain and batched.
+//
+// Reference kernels replaced (rocmarchive/hcBLAS):
+//   gemv_TransA    lib/src/blas/sgemv/sgemv_array.cpp:30-147  (one WG / column)
+//   gemv_NoTransA  lib/src/blas/sgemv/sgemv_array.cpp:520-563 (one thread / row)
+//   gemv_alpha0_*  lib/src/blas/sgemv/sgemv_array.cpp:709-825
+//   ger_HC         lib/src/blas/sger/sger_array.cpp:27-98     (one element / thread)
+//
+// Every matrix is handled as column-major STORAGE S (R contiguous rows, Cc
+// columns, leading dimension ld). Row-major A is column-major A^T, so
+//   col-major N, row-major T  ->  "panel" kernel:  y[r] = sum_c S[r,c] x[c]
+//   col-major T, row-major N  ->  "column" kernel: y[c] = sum_r S[r,c] x[r]
+// Both kernels stream S once with 16-byte loads and split the reduction
+// across workgroups when the output alone cannot fill 256 CUs; split partials
+// go to a per-handle workspace and the last workgroup of each output block
+// (ticket counter) sums them in split order, so results are deterministic.
+//
+// GER updates S[r,c] = fma(v[r], alpha*w[c], S[r,c]) where w is the vector
+// that indexes storage columns; with v = x, w = y for column-major and
+// v = y, w = x for row-major this is bit-identical to CBLAS (netlib
+// "temp = alpha*y(j); a(i,j) += x(i)*temp", row-major by argument swap).
+#include "common.hpp"
+
+namespace hcb {
+namespace {
+
+constexpr int kThreads = 256;
+
+template <typename T>
+struct VecT;
+template <>
+struct VecT<float> {
+  using type = float4;
+  static constexpr int n = 4;
+};
+template <>
+struct VecT<double> {
+  using type = double2;
+  static constexpr int n = 2;
+};
+
+template <typename T>
+__device__ __forceinline__ T vlane(const typename VecT<T>::type &v, int i) {
+  return reinterpret_cast<const T *>(&v)[i];
+}
+
+__device__ __forceinline__ int64_t bidx(int64_t i, int64_t n, int64_t inc) {
+  return inc > 0 ? i * inc : (n - 1 - i) * (-inc);
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T fma_t(T a, T b, T c);
+template <>
+__device__ __forceinline__ float fma_t(float a, float b, float c) {
+  return __builtin_fmaf(a, b, c);
+}
+template <>
+__device__ __forceinline__ double fma_t(double a, double b, double c) {
+  return __builtin_fma(a, b, c);
+}
+
+struct GemvShape {
+  int64_t R, Cc, ld;    // storage rows (contiguous), columns, leading dim
+  int64_t incx, incy;   // BLAS increments of x (reduction) and y (output)
+  int64_t sA, sx, sy;   // batch strides (elements)
+  int splits;           // reduction split count (>= 1)
+};
+
+// Final value of one output element: y = alpha*sum + beta*y; beta == 0 does
+// not read y (CBLAS: y need not be initialised).
+template <typename T>
+__device__ __forceinline__ void store_y(T *y, T sum, T alpha, T beta) {
+  T r = alpha * sum;
+  if (beta != T(0)) r = fma_t(beta, *y, r);
+  *y = r;
+}
+
+// Publishes `count` partial sums of this workgroup (split `split`) and, when it
+// is the last of `splits` workgroups for this output block, reduces them in
+// split order and writes y. Returns after all work for the block is done.
+template <typename T>
+__device__ void split_finish(T *my_vals, int count, int64_t out0,
+                             int64_t len_y, T *part, unsigned int *ticket,
+                             int split, int splits, T *y, int64_t incy,
+                             T alpha, T beta) {
+  __shared__ bool last;
+  // my_vals lives in LDS and holds `count` values for outputs out0..
+  for (int i = threadIdx.x; i < count; i += kThreads)
+    if (out0 + i < len_y)
+      __hip_atomic_store(&part[(int64_t)split * len_y + out0 + i], my_vals[i],
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned int t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL,
+                                            __HIP_MEMORY_SCOPE_AGENT);
+    last = (t == (unsigned)splits - 1);
+  }
+  __syncthreads();
+  if (!last) return;
+  for (int i = threadIdx.x; i < count; i += kThreads) {
+    int64_t o = out0 + i;
+    if (o >= len_y) continue;
+    T s = T(0);
+    for (int k = 0; k < splits; ++k)
+      s += __hip_atomic_load(&part[(int64_t)k * len_y + o], __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+    store_y(&y[bidx(o, len_y, incy)], s, alpha, beta);
+  }
+  if (threadIdx.x == 0)
+    __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---------------------------------------------------------------------------
+// Panel kernel: y[r] = alpha * sum_c S[r,c] x[c] + beta*y[r].
+// Workgroup = 4 waves; each wave owns 64*VN consecutive rows (VN = values per
+// 16-byte load); blockIdx.y selects a contiguous range of columns (split).
+// ---------------------------------------------------------------------------
+template <typename T, bool VEC>
+__global__ __launch_bounds__(kThreads) void gemv_panel_kernel(
+    GemvShape s, const T *__restrict__ A, const T *__restrict__ x, T *y,
+    T alpha, T beta, T *part, unsigned int *tickets) {
+  constexpr int VN = VEC ? VecT<T>::n : 1;
+  constexpr int ROWS = kThreads * VN;
+  __shared__ T vals[ROWS];
+  const int64_t z = blockIdx.z;
+  A += z * s.sA;
+  x += z * s.sx;
+  y += z * s.sy;
+  const int64_t r0 = (int64_t)blockIdx.x * ROWS + (int64_t)threadIdx.x * VN;
+  const int split = blockIdx.y;
+  const int64_t cper = cdiv_dev(s.Cc, s.splits);
+  const int64_t c_begin = split * cper;
+  const int64_t c_end = c_begin + cper < s.Cc ? c_begin + cper : s.Cc;
+
+  T acc[VN];
+#pragma unroll
+  for (int e = 0; e < VN; ++e) acc[e] = T(0);
+
+  if (VEC) {
+    using V = typename VecT<T>::type;
+    if (r0 < s.R) {  // R % VN == 0 on this path
+      const T *col = A + r0;
+      int64_t c = c_begin;
+      constexpr int U = 8;
+      for (; c + U <= c_end; c += U) {
+        V a[U];
+        T xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          a[u] = *reinterpret_cast<const V *>(col + (c + u) * s.ld);
+          xv[u] = x[bidx(c + u, s.Cc, s.incx)];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+          for (int e = 0; e < VN; ++e)
+            acc[e] = fma_t(vlane<T>(a[u], e), xv[u], acc[e]);
+      }
+      for (; c < c_end; ++c) {
+        V a = *reinterpret_cast<const V *>(col + c * s.ld);
+        T xv = x[bidx(c, s.Cc, s.incx)];
+#pragma unroll
+        for (int e = 0; e < VN; ++e) acc[e] = fma_t(vlane<T>(a, e), xv, acc[e]);
+      }
+    }
+  } else {
+    if (r0 < s.R) {
+      const T *col = A + r0;
+      int64_t c = c_begin;
+      for (; c + 4 <= c_end; c += 4) {
+        T a0 = col[c * s.ld], a1 = col[(c + 1) * s.ld];
+        T a2 = col[(c + 2) * s.ld], a3 = col[(c + 3) * s.ld];
+        acc[0] = fma_t(a0, x[bidx(c, s.Cc, s.incx)], acc[0]);
+        acc[0] = fma_t(a1, x[bidx(c + 1, s.Cc, s.incx)], acc[0]);
+        acc[0] = fma_t(a2, x[bidx(c + 2, s.Cc, s.incx)], acc[0]);
+        acc[0] = fma_t(a3, x[bidx(c + 3, s.Cc, s.incx)], acc[0]);
+      }
+      for (; c < c_end; ++c)
+        acc[0] = fma_t(col[c * s.ld], x[bidx(c, s.Cc, s.incx)], acc[0]);
+    }
+  }
+
+  if (s.splits == 1) {
+#pragma unroll
+    for (int e = 0; e < VN; ++e)
+      if (r0 + e < s.R)
+        store_y(&y[bidx(r0 + e, s.R, s.incy)], acc[e], alpha, beta);
+    return;
+  }
+#pragma unroll
+  for (int e = 0; e < VN; ++e) vals[threadIdx.x * VN + e] = acc[e];
+  __syncthreads();
+  split_finish<T>(vals, ROWS, (int64_t)blockIdx.x * ROWS, s.R,
+                  part + z * s.splits * s.R,
+                  tickets + z * gridDim.x + blockIdx.x, split, s.splits, y,
+                  s.incy, alpha, beta);
+}
+
+// ---------------------------------------------------------------------------
+// Column kernel: y[c] = alpha * sum_r S[r,c] x[r] + beta*y[c].
+// Workgroup = 4 waves x CPW columns; each wave reduces CPW columns over its
+// row range with 16-byte loads, reusing each x load for the CPW columns.
+// blockIdx.y selects a contiguous range of rows (split).
+// ---------------------------------------------------------------------------
+constexpr int kCPW = 4;
+constexpr int kColsPerBlock = 4 * kCPW;
+
+template <typename T, bool VEC>
+__global__ __launch_bounds__(kThreads) void gemv_column_kernel(
+    GemvShape s, const T *__restrict__ A, const T *__restrict__ x, T *y,
+    T alpha, T beta, T *part, unsigned int *tickets) {
+  constexpr int VN = VEC ? VecT<T>::n : 1;
+  __shared__ T vals[kColsPerBlock];
+  const int64_t z = blockIdx.z;
+  A += z * s.sA;
+  x += z * s.sx;
+  y += z * s.sy;
+  const int wave = threadIdx.x >> 6, lane_id = threadIdx.x & 63;
+  const int64_t cbase = (int64_t)blockIdx.x * kColsPerBlock + wave * kCPW;
+  const int split = blockIdx.y;
+  // Row range of this split, a multiple of the 64*VN step.
+  const int64_t step = 64 * VN;
+  const int64_t rsteps = cdiv_dev(s.R, step);
+  const int64_t sper = cdiv_dev(rsteps, s.splits);
+  const int64_t r_begin = split * sper * step;
+  int64_t r_end = r_begin + sper * step;
+  if (r_end > s.R) r_end = s.R;
+
+  T acc[kCPW][VN];
+#pragma unroll
+  for (int j = 0; j < kCPW; ++j)
+#pragma unroll
+    for (int e = 0; e < VN; ++e) acc[j][e] = T(0);
+
+  const T *cols[kCPW];
+  bool colok[kCPW];
+#pragma unroll
+  for (int j = 0; j < kCPW; ++j) {
+    colok[j] = cbase + j < s.Cc;
+    cols[j] = A + (colok[j] ? (cbase + j) : 0) * s.ld;
+  }
+
+  if (VEC) {
+    using V = typename VecT<T>::type;
+    const bool xunit = s.incx == 1;
+    for (int64_t r = r_begin + lane_id * VN; r < r_end; r += step) {
+      // On this path R % VN == 0, so a vector never straddles R.
+      T xv[VN];
+      if (xunit) {
+        V t = *reinterpret_cast<const V *>(x + r);
+#pragma unroll
+        for (int e = 0; e < VN; ++e) xv[e] = vlane<T>(t, e);
+      } else {
+#pragma unroll
+        for (int e = 0; e < VN; ++e) xv[e] = x[bidx(r + e, s.R, s.incx)];
+      }
+#pragma unroll
+      for (int j = 0; j < kCPW; ++j) {
+        if (!colok[j]) continue;
+        V a = *reinterpret_cast<const V *>(cols[j] + r);
+#pragma unroll
+        for (int e = 0; e < VN; ++e) acc[j][e] = fma_t(vlane<T>(a, e), xv[e], acc[j][e]);
+      }
+    }
+  } else {
+    for (int64_t r = r_begin + lane_id; r < r_end; r += step) {
+      T xv = x[bidx(r, s.R, s.incx)];
+#pragma unroll
+      for (int j = 0; j < kCPW; ++j)
+        if (colok[j]) acc[j][0] = fma_t(cols[j][r], xv, acc[j][0]);
+    }
+  }
+
+  T sums[kCPW];
+#pragma unroll
+  for (int j = 0; j < kCPW; ++j) {
+    T t = T(0);
+#pragma unroll
+    for (int e = 0; e < VN; ++e) t += acc[j][e];
+    sums[j] = wave_sum(t);
+  }
+
+  if (s.splits == 1) {
+    if (lane_id == 0) {
+#pragma unroll
+      for (int j = 0; j < kCPW; ++j)
+        if (colok[j])
+          store_y(&y[bidx(cbase + j, s.Cc, s.incy)], sums[j], alpha, beta);
+    }
+    return;
+  }
+  if (lane_id == 0) {
+#pragma unroll
+    for (int j = 0; j < kCPW; ++j) vals[wave * kCPW + j] = sums[j];
+  }
+  __syncthreads();
+  split_finish<T>(vals, kColsPerBlock, (int64_t)blockIdx.x * kColsPerBlock,
+                  s.Cc, part + z * s.splits * s.Cc,
+                  tickets + z * gridDim.x + blockIdx.x, split, s.splits, y,
+                  s.incy, alpha, beta);
+}
+
+// alpha == 0 (or an empty reduction): y = beta*y without reading A or x
+// (reference gemv_alpha0_col, sgemv_array.cpp:709-734).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void gemv_scale_y_kernel(
+    int64_t len, int64_t incy, int64_t sy, T *y, T beta) {
+  y += (int64_t)blockIdx.z * sy;
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < len;
+       i += (int64_t)gridDim.x * kThreads) {
+    T *p = &y[bidx(i, len, incy)];
+    *p = beta == T(0) ? T(0) : beta * *p;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// GER: S[r,c] = fma(v[r], alpha*w[c], S[r,c]).
+// Each thread owns VN consecutive rows (one 16-byte vector) and walks kGerCols
+// columns, so v stays in registers and every store is a full 16-byte store.
+// ---------------------------------------------------------------------------
+constexpr int kGerCols = 16;
+
+template <typename T, bool VEC>
+__global__ __launch_bounds__(kThreads) void ger_kernel(
+    int64_t R, int64_t Cc, T *A, int64_t ld, int64_t sA,
+    const T *__restrict__ v, int64_t incv, int64_t sv,
+    const T *__restrict__ w, int64_t incw, int64_t sw, T alpha) {
+  constexpr int VN = VEC ? VecT<T>::n : 1;
+  const int64_t z = blockIdx.z;
+  A += z * sA;
+  v += z * sv;
+  w += z * sw;
+  const int64_t r0 = ((int64_t)blockIdx.x * kThreads + threadIdx.x) * VN;
+  if (r0 >= R) return;
+  const int64_t c0 = (int64_t)blockIdx.y * kGerCols;
+  T vr[VN];
+#pragma unroll
+  for (int e = 0; e < VN; ++e)
+    vr[e] = (r0 + e < R) ? v[bidx(r0 + e, R, incv)] : T(0);
+  const int64_t cend = c0 + kGerCols < Cc ? c0 + kGerCols : Cc;
+  if (VEC) {
+    using V = typename VecT<T>::type;
+    for (int64_t c = c0; c < cend; ++c) {
+      const T t = alpha * w[bidx(c, Cc, incw)];
+      V *p = reinterpret_cast<V *>(A + r0 + c * ld);
+      V a = *p;
+      T *ae = reinterpret_cast<T *>(&a);
+#pragma unroll
+      for (int e = 0; e < VN; ++e) ae[e] = fma_t(vr[e], t, ae[e]);
+      *p = a;
+    }
+  } else {
+    for (int64_t c = c0; c < cend; ++c) {
+      const T t = alpha * w[bidx(c, Cc, incw)];
+      T *p = A + r0 + c * ld;
+      *p = fma_t(vr[0], t, *p);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+template <typename T>
+hcblasStatus_t run_gemv(hcblasHandle_t h, bool panel, int64_t R, int64_t Cc,
+                        T alpha, const T *A, int64_t ld, const T *x,
+                        int64_t incx, T beta, T *y, int64_t incy, int64_t sA,
+                        int64_t sx, int64_t sy, int64_t batch) {
+  const int64_t len_y = panel ? R : Cc;
+  const int64_t len_x = panel ? Cc : R;
+  if (len_y == 0 || batch == 0) return HCBLAS_STATUS_SUCCESS;
+  if (alpha == T(0) || len_x == 0) {
+    if (beta == T(1)) return HCBLAS_STATUS_SUCCESS;
+    dim3 g((unsigned)std::min<int64_t>(cdiv(len_y, kThreads), 4096), 1,
+           (unsigned)batch);
+    gemv_scale_y_kernel<T><<<g, kThreads, 0, h->stream>>>(len_y, incy, sy, y,
+                                                          beta);
+    return launch_status();
+  }
+  constexpr int VN = VecT<T>::n;
+  // 16-byte path: aligned base, leading dimension and rows a multiple of VN.
+  const bool vec = aligned16(A) && (ld % VN == 0) && (R % VN == 0) &&
+                   (panel || incx != 1 || aligned16(x));
+  const int64_t target = (int64_t)h->num_cus * 4;  // workgroups to aim for
+  GemvShape s{R, Cc, ld, incx, incy, sA, sx, sy, 1};
+  int64_t oblocks;
+  if (panel) {
+    oblocks = cdiv(R, (int64_t)kThreads * (vec ? VN : 1));
+    int64_t max_split = std::max<int64_t>(1, Cc / 64);
+    s.splits = (int)std::min<int64_t>(
+        std::max<int64_t>(1, cdiv(target, oblocks * batch)), max_split);
+  } else {
+    oblocks = cdiv(Cc, kColsPerBlock);
+    int64_t rsteps = cdiv(R, 64 * (vec ? VN : 1));
+    int64_t max_split = std::max<int64_t>(1, rsteps / 16);
+    s.splits = (int)std::min<int64_t>(
+        std::max<int64_t>(1, cdiv(target, oblocks * batch)), max_split);
+  }
+  if (s.splits > 256) s.splits = 256;
+  T *part = nullptr;
+  unsigned int *tickets = nullptr;
+  if (s.splits > 1) {
+    size_t pbytes = sizeof(T) * (size_t)s.splits * (size_t)len_y * batch;
+    if (ensure_workspace(h->gemv_partials, pbytes) != hipSuccess)
+      return HCBLAS_STATUS_ALLOC_FAILED;
+    if (ensure_tickets(h, (size_t)oblocks * batch) != hipSuccess)
+      return HCBLAS_STATUS_ALLOC_FAILED;
+    part = static_cast<T *>(h->gemv_partials.ptr);
+    tickets = static_cast<unsigned int *>(h->gemv_tickets.ptr);
+  }
+  for (int64_t b0 = 0; b0 < batch; b0 += 65535) {
+    int64_t nb = std::min<int64_t>(65535, batch - b0);
+    dim3 grid((unsigned)oblocks, (unsigned)s.splits, (unsigned)nb);
+    const T *Ab = A + b0 * sA;
+    const T *xb = x + b0 * sx;
+    T *yb = y + b0 * sy;
+    if (panel) {
+      if (vec)
+        gemv_panel_kernel<T, true><<<grid, kThreads, 0, h->stream>>>(
+            s, Ab, xb, yb, alpha, beta, part, tickets);
+      else
+        gemv_panel_kernel<T, false><<<grid, kThreads, 0, h->stream>>>(
+            s, Ab, xb, yb, alpha, beta, part, tickets);
+    } else {
+      if (vec)
+        gemv_column_kernel<T, true><<<grid, kThreads, 0, h->stream>>>(
+            s, Ab, xb, yb, alpha, beta, part, tickets);
+      else
+        gemv_column_kernel<T, false><<<grid, kThreads, 0, h->stream>>>(
+            s, Ab, xb, yb, alpha, beta, part, tickets);
+    }
+    hcblasStatus_t st = launch_status();
+    if (st != HCBLAS_STATUS_SUCCESS) return st;
+  }
+  return HCBLAS_STATUS_SUCCESS;
+}
+
+// Common argument handling for gemv in the reference's (order, trans) terms.
+template <typename T>
+hcblasStatus_t gemv_entry(hcblasHandle_t h, hcblasOrder_t order,
+                          hcblasOperation_t trans, int M, int N,
+                          const T *alpha, const T *A, int64_t aOffset,
+                          int64_t lda, const T *x, int64_t xOffset,
+                          int64_t incx, const T *beta, T *y, int64_t yOffset,
+                          int64_t incy, int64_t batch, bool ref_batch_strides) {
+  if (!handle_ok(h)) return HCBLAS_STATUS_NOT_INITIALIZED;
+  if (M < 0 || N < 0 || incx == 0 || incy == 0 || batch < 0)
+    return HCBLAS_STATUS_INVALID_VALUE;
+  if (order != HCBLAS_ROW_MAJOR && order != HCBLAS_COL_MAJOR)
+    return HCBLAS_STATUS_INVALID_VALUE;
+  if (trans != HCBLAS_OP_N && trans != HCBLAS_OP_T && trans != HCBLAS_OP_C)
+    return HCBLAS_STATUS_INVALID_VALUE;
+  const bool notrans = trans == HCBLAS_OP_N;
+  const int64_t len_x = notrans ? N : M, len_y = notrans ? M : N;
+  // Storage is column-major R x Cc.
+  const bool col = order == HCBLAS_COL_MAJOR;
+  const int64_t R = col ? M : N, Cc = col ? N : M;
+  if (lda < std::max<int64_t>(1, R)) return HCBLAS_STATUS_INVALID_VALUE;
+  if (M == 0 || N == 0 || batch == 0) {
+    if (len_y == 0 || batch == 0) return HCBLAS_STATUS_SUCCESS;
+  }
+  if (alpha == nullptr || beta == nullptr || y == nullptr)
+    return HCBLAS_STATUS_INVALID_VALUE;
+  if (*alpha != T(0) && len_x > 0 && (A == nullptr || x == nullptr))
+    return HCBLAS_STATUS_INVALID_VALUE;
+  // Batch strides as in hcblas.cpp:1008-1019: x by len(x), y by len(y), A by
+  // M*N.
+  int64_t sA = ref_batch_strides ? (int64_t)M * N : 0;
+  int64_t sx = ref_batch_strides ? len_x : 0;
+  int64_t sy = ref_batch_strides ? len_y : 0;
+  const bool panel = col == notrans;
+  return run_gemv<T>(h, panel, R, Cc, *alpha, A ? A + aOffset : nullptr, lda,
+                     x ? x + xOffset : nullptr, incx, *beta, y + yOffset, incy,
+                     sA, sx, sy, batch);
+}
+
+template <typename T>
+hcblasStatus_t ger_entry(hcblasHandle_t h, hcblasOrder_t order, int M, int N,
+                         const T *alpha, const T *x, int64_t xOffset,
+                         int64_t incx, const T *y, int64_t yOffset,
+                         int64_t incy, T *A, int64_t aOffset, int64_t lda,
+                         int64_t batch) {
+  if (!handle_ok(h)) return HCBLAS_STATUS_NOT_INITIALIZED;
+  if (M < 0 || N < 0 || incx == 0 || incy == 0 || batch < 0)
+    return HCBLAS_STATUS_INVALID_VALUE;
+  if (order != HCBLAS_ROW_MAJOR && order != HCBLAS_COL_MAJOR)
+    return HCBLAS_STATUS_INVALID_VALUE;
+  const bool col = order == HCBLAS_COL_MAJOR;
+  const int64_t R = col ? M : N, Cc = col ? N : M;
+  if (lda < std::max<int64_t>(1, R)) return HCBLAS_STATUS_INVALID_VALUE;
+  if (M == 0 || N == 0 || batch == 0) return HCBLAS_STATUS_SUCCESS;
+  if (alpha == nullptr || x == nullptr || y == nullptr || A == nullptr)
+    return HCBLAS_STATUS_INVALID_VALUE;
+  if (*alpha == T(0)) return HCBLAS_STATUS_SUCCESS;  // netlib quick return
+  // Batch strides as in hcblas.cpp:1160-1167: x by M, y by N, A by M*N.
+  const int64_t sA = batch > 1 ? (int64_t)M * N : 0;
+  const int64_t sxv = batch > 1 ? M : 0, syv = batch > 1 ? N : 0;
+  const T *v = col ? x + xOffset : y + yOffset;
+  const T *w = col ? y + yOffset : x + xOffset;
+  const int64_t incv = col ? incx : incy, incw = col ? incy : incx;
+  const int64_t sv = col ? sxv : syv, sw = col ? syv : sxv;
+  T *Ab = A + aOffset;
+  constexpr int VN = VecT<T>::n;
+  const bool vec = aligned16(Ab) && (lda % VN == 0) && (R % VN == 0) &&
+                   (batch == 1 || sA % VN == 0);
+  const int64_t rows_per_block = (int64_t)kThreads * (vec ? VN : 1);
+  for (int64_t b0 = 0; b0 < batch; b0 += 65535) {
+    int64_t nb = std::min<int64_t>(65535, batch - b0);
+    dim3 grid((unsigned)cdiv(R, rows_per_block),
+              (unsigned)cdiv(Cc, kGerCols), (unsigned)nb);
+    if (vec)
+      ger_kernel<T, true><<<grid, kThreads, 0, h->stream>>>(
+          R, Cc, Ab + b0 * sA, lda, sA, v + b0 * sv, incv, sv, w + b0 * sw,
+          incw, sw, *alpha);
+    else
+      ger_kernel<T, false><<<grid, kThreads, 0, h->stream>>>(
+          R, Cc, Ab + b0 * sA, lda, sA, v + b0 * sv, incv, sv, w + b0 * sw,
+          incw, sw, *alpha);
+    hcblasStatus_t st = launch_status();
+    if (st != HCBLAS_STATUS_SUCCESS) return st;
+  }
+  return HCBLAS_STATUS_SUCCESS;
+}
+
+}  // namespace
+}  // namespace hcb
+
+using namespace hcb;
+
+extern "C" {
+
+hcblasStatus_t hcblasSgemv(hcblasHandle_t handle, hcblasOperation_t trans,
+                           int m, int n, const float *alpha, const float *A,
+                           int lda, const float *x, int incx, const float *beta,
+                           float *y, int incy) {
+  return gemv_entry<float>(handle, HCBLAS_COL_MAJOR, trans, m, n, alpha, A, 0,
+                           lda, x, 0, incx, beta, y, 0, incy, 1, false);
+}
+hcblasStatus_t hcblasDgemv(hcblasHandle_t handle, hcblasOperation_t trans,
+                           int m, int n, const double *alpha, const double *A,
+                           int lda, const double *x, int incx,
+                           const double *beta, double *y, int incy) {
+  return gemv_entry<double>(handle, HCBLAS_COL_MAJOR, trans, m, n, alpha, A, 0,
+                            lda, x, 0, incx, beta, y, 0, incy, 1, false);
+}
+hcblasStatus_t hcblasSgemvBatched(hcblasHandle_t handle,
+                                  hcblasOperation_t trans, int m, int n,
+                                  const float *alpha, const float *A, int lda,
+                                  const float *x, int incx, const float *beta,
+                                  float *y, int incy, int batchCount) {
+  return gemv_entry<float>(handle, HCBLAS_COL_MAJOR, trans, m, n, alpha, A, 0,
+                           lda, x, 0, incx, beta, y, 0, incy, batchCount, true);
+}
+hcblasStatus_t hcblasDgemvBatched(hcblasHandle_t handle,
+                                  hcblasOperation_t trans, int m, int n,
+                                  const double *alpha, const double *A,
+                                  int lda, const double *x, int incx,
+                                  const double *beta, double *y, int incy,
+                                  int batchCount) {
+  return gemv_entry<double>(handle, HCBLAS_COL_MAJOR, trans, m, n, alpha, A, 0,
+                            lda, x, 0, incx, beta, y, 0, incy, batchCount,
+                            true);
+}
+hcblasStatus_t hcblas_sgemv(hcblasHandle_t handle, hcblasOrder_t order,
+                            hcblasOperation_t type, int M, int N,
+                            const float *alpha, const float *A,
+                            int64_t aOffset, int64_t lda, const float *X,
+                            int64_t xOffset, int64_t incX, const float *beta,
+                            float *Y, int64_t yOffset, int64_t incY) {
+  return gemv_entry<float>(handle, order, type, M, N, alpha, A, aOffset, lda,
+                           X, xOffset, incX, beta, Y, yOffset, incY, 1, false);
+}
+hcblasStatus_t hcblas_dgemv(hcblasHandle_t handle, hcblasOrder_t order,
+                            hcblasOperation_t type, int M, int N,
+                            const double *alpha, const double *A,
+                            int64_t aOffset, int64_t lda, const double *X,
+                            int64_t xOffset, int64_t incX, const double *beta,
+                            double *Y, int64_t yOffset, int64_t incY) {
+  return gemv_entry<double>(handle, order, type, M, N, alpha, A, aOffset, lda,
+                            X, xOffset, incX, beta, Y, yOffset, incY, 1, false);
+}
+
+hcblasStatus_t hcblasSger(hcblasHandle_t handle, int m, int n,
+                          const float *alpha, const float *x, int incx,
+                          const float *y, int incy, float *A, int lda) {
+  return ger_entry<float>(handle, HCBLAS_COL_MAJOR, m, n, alpha, x, 0, incx, y,
+                          0, incy, A, 0, lda, 1);
+}
+hcblasStatus_t hcblasDger(hcblasHandle_t handle, int m, int n,
+                          const double *alpha, const double *x, int incx,
+                          const double *y, int incy, double *A, int lda) {
+  return ger_entry<double>(handle, HCBLAS_COL_MAJOR, m, n, alpha, x, 0, incx, y,
+                           0, incy, A, 0, lda, 1);
+}
+hcblasStatus_t hcblasSgerBatched(hcblasHandle_t handle, int m, int n,
+                                 const float *alpha, const float *x, int incx,
+                                 const float *y, int incy, float *A, int lda,
+                                 int batchCount) {
+  return ger_entry<float>(handle, HCBLAS_COL_MAJOR, m, n, alpha, x, 0, incx, y,
+                          0, incy, A, 0, lda, batchCount);
+}
+hcblasStatus_t hcblasDgerBatched(hcblasHandle_t handle, int m, int n,
+                                 const double *alpha, const double *x,
+                                 int incx, const double *y, int incy,
+                                 double *A, int lda, int batchCount) {
+  return ger_entry<double>(handle, HCBLAS_COL_MAJOR, m, n, alpha, x, 0, incx,
+                           y, 0, incy, A, 0, lda, batchCount);
+}
+hcblasStatus_t hcblas_sger(hcblasHandle_t handle, hcblasOrder_t order, int M,
+                           int N, const float *alpha, const float *X,
+                           int64_t xOffset, int64_t incX, const float *Y,
+                           int64_t yOffset, int64_t incY, float *A,
+                           int64_t aOffset, int64_t lda) {
+  return ger_entry<float>(handle, order, M, N, alpha, X, xOffset, incX, Y,
+                          yOffset, incY, A, aOffset, lda, 1);
+}
+hcblasStatus_t hcblas_dger(hcblasHandle_t handle, hcblasOrder_t order, int M,
+                           int N, const double *alpha, const double *X,
+                           int64_t xOffset, int64_t incX, const double *Y,
+                           int64_t yOffset, int64_t incY, double *A,
+                           int64_t aOffset, int64_t lda) {
+  return ger_entry<double>(handle, order, M, N, alpha, X, xOffset, incX, Y,
+                           yOffset, incY, A, aOffset, lda, 1);
+}
+
+}  // extern "C"

@@ -1,0 +1,314 @@
+"""GEMM parity on the GPU through the C ABI (S/D/C/Z/H, plain, pointer-array
+and strided batches, explicit order and offsets).
+
+Reference tests mirrored: test/unit/hcblas_{s,d,c,z}gemm.cpp (the 189x9x19
+order x trans sweep on integer data, exact vs CBLAS), test/unit-api/
+test-blas.cpp (123x78x23 closed form, pointer-array batches of 32), plus
+seeded real-valued problems across tile boundaries. Real-valued results must
+lie within c*K*eps*(|alpha|*sum|a||b| + |beta||c|) of the FP64 /
+double-double value (c = 2).
+"""
+import numpy as np
+import pytest
+
+from gpu_util import bitwise_equal, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+ROW, COL = 0, 1
+N_, T_, C_ = 0, 1, 2
+DT = {"s": np.float32, "d": np.float64, "c": np.complex64, "z": np.complex128}
+
+
+def _run(hc, handle, p, order, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta,
+         C, ldc, aoff=0, boff=0, coff=0):
+    api = hc.api
+    dA, dB, dC = to_dev(A), to_dev(B), to_dev(C)
+    if p in "sd":
+        st = getattr(api, f"hcblas_{p}gemm")(handle, order, ta, tb, M, N, K,
+                                             alpha, dA, lda, dB, ldb, beta, dC,
+                                             ldc, aoff, boff, coff)
+    else:
+        st = getattr(api, f"hcblas_{p}gemm")(handle, order, ta, tb, M, N, K,
+                                             alpha, dA, aoff, lda, dB, boff,
+                                             ldb, beta, dC, coff, ldc)
+    assert st == 0
+    return to_host(dC)
+
+
+@pytest.mark.parametrize("p", ["s", "d", "c", "z"])
+def test_reference_recipe_bit_exact(hc, handle, golden, p):
+    g = golden("gemm_ref_int.npz")
+    M, N, K = (int(v) for v in g["dims"])
+    ncases = 8 if p in "sd" else 11
+    ab = 1.0 if p in "sd" else 1 + 1j
+    for i in range(ncases):
+        o, ta, tb, lda, ldb, ldc = (int(v) for v in g[f"{p}_case{i}"])
+        got = _run(hc, handle, p, o, ta, tb, M, N, K, ab, g[f"{p}_A"], lda,
+                   g[f"{p}_B"], ldb, ab, g[f"{p}_C"], ldc)
+        assert bitwise_equal(got, g[f"{p}_expect{i}"]), f"case {i}"
+
+
+@pytest.mark.parametrize("p", ["s", "d", "c", "z"])
+def test_real_valued_golden(hc, handle, golden, orc, p):
+    g = golden("real_valued.npz")
+    M, N, K = (int(v) for v in g["dims"])
+    a, b = g[f"{p}_ab"]
+    if p in "sd":
+        a, b = float(a.real), float(b.real)
+    else:
+        a, b = complex(a), complex(b)
+    eps = np.finfo(DT[p]).eps
+    for i in range(8):
+        o, ta, tb, lda, ldb, ldc = (int(v) for v in g[f"{p}_case{i}"])
+        got = _run(hc, handle, p, o, ta, tb, M, N, K, a, g[f"{p}_A"], lda,
+                   g[f"{p}_B"], ldb, b, g[f"{p}_C"], ldc)
+        exp = g[f"{p}_expect{i}"]
+        assert np.max(np.abs(got - exp)) <= 4 * K * eps * 4, f"case {i}"
+
+
+def test_closed_forms(hc, handle):
+    # test-blas.cpp:1861-1920 (A=B=C=1, alpha=beta=2, 123x78x23 -> 2K+2)
+    M, N, K = 123, 78, 23
+    for p in "sdcz":
+        dt = DT[p]
+        for o in (COL, ROW):
+            lda, ldb, ldc = (M, K, M) if o == COL else (K, N, N)
+            got = _run(hc, handle, p, o, N_, N_, M, N, K, 2, np.ones(M * K, dt),
+                       lda, np.ones(K * N, dt), ldb, 2, np.ones(M * N, dt), ldc)
+            assert np.all(got == 2 * K + 2), (p, o)
+    # test-hipblas.cpp:1815-1853 (A=1, B=2, C=3, alpha=beta=1 -> 2K+3)
+    api = hc.api
+    for o in (COL, ROW):
+        lda, ldb, ldc = (M, K, M) if o == COL else (K, N, N)
+        dC = to_dev(np.full(M * N, 3, np.uint16) * 0 + hc.half_bits(3.0))
+        st = api.hcblas_hgemm(handle, o, N_, N_, M, N, K, 1.0,
+                              to_dev(np.full(M * K, hc.half_bits(1.0), np.uint16)),
+                              lda,
+                              to_dev(np.full(K * N, hc.half_bits(2.0), np.uint16)),
+                              ldb, 1.0, dC, ldc, 0, 0, 0)
+        assert st == 0
+        got = to_host(dC).view(np.uint16).view(np.float16)
+        assert np.all(got == 2 * K + 3)
+
+
+def _rand(orc, p, n, seed):
+    if p == "s":
+        return orc.uniform_f32(n, seed)
+    if p == "d":
+        return orc.uniform_f64(n, seed)
+    gen = orc.uniform_f32 if p == "c" else orc.uniform_f64
+    v = gen(2 * n, seed)
+    return (v[0::2] + 1j * v[1::2]).astype(DT[p])
+
+
+def _check_against_exact(orc, p, o, ta, tb, M, N, K, alpha, A, lda, B, ldb,
+                         beta, C, ldc, got, nsamp=512):
+    rng = np.random.default_rng(M * 131 + N * 7 + K)
+    ns = min(nsamp, M * N)
+    rows = rng.integers(0, M, ns)
+    cols = rng.integers(0, N, ns)
+    ref = orc.gemm_sample_exact(o, ta, tb, M, N, K, alpha, A, lda, B, ldb,
+                                beta, C, ldc, rows, cols)
+    idx = rows + cols * ldc if o == COL else rows * ldc + cols
+    eps = np.finfo(DT[p]).eps
+    # |a|,|b|,|c| <= 1 (complex: <= sqrt 2)
+    mag = 2.0 if p in "cz" else 1.0
+    tol = 2 * (K + 2) * eps * (abs(alpha) * K * mag + abs(beta) * mag + 1)
+    err = np.abs(got[idx] - ref)
+    assert np.all(np.isfinite(got[idx]))
+    assert err.max() <= tol, (err.max(), tol)
+
+
+SHAPES = [(1, 1, 1), (5, 3, 7), (16, 16, 16), (64, 64, 64), (65, 63, 67),
+          (128, 128, 128), (129, 250, 33), (300, 17, 515), (512, 384, 96),
+          (1000, 1000, 8)]
+
+
+@pytest.mark.parametrize("p", ["s", "d", "c", "z"])
+@pytest.mark.parametrize("M,N,K", SHAPES)
+def test_seeded_shapes_all_cases(hc, handle, orc, p, M, N, K):
+    transes = (N_, T_) if p in "sd" else (N_, T_, C_)
+    alpha = 0.75 if p in "sd" else 0.75 - 0.5j
+    beta = -1.25 if p in "sd" else -1.25 + 0.25j
+    seed = M * 1000 + N * 10 + K
+    for o in (COL, ROW):
+        for ta in transes:
+            for tb in transes:
+                if p in "cz" and (ta, tb) not in ((N_, N_), (T_, C_), (C_, N_),
+                                                  (C_, C_), (N_, T_)):
+                    continue
+                ar, ac = (M, K) if ta == N_ else (K, M)
+                br, bc = (K, N) if tb == N_ else (N, K)
+                # stored rows (leading dimension covers them), plus padding
+                lda = (ar if o == COL else ac) + 2
+                ldb = (br if o == COL else bc) + 1
+                ldc = (M if o == COL else N) + 3
+                A = _rand(orc, p, lda * (ac if o == COL else ar), seed + 1)
+                B = _rand(orc, p, ldb * (bc if o == COL else br), seed + 2)
+                C = _rand(orc, p, ldc * (N if o == COL else M), seed + 3)
+                got = _run(hc, handle, p, o, ta, tb, M, N, K, alpha, A, lda, B,
+                           ldb, beta, C, ldc)
+                _check_against_exact(orc, p, o, ta, tb, M, N, K, alpha, A, lda,
+                                     B, ldb, beta, C, ldc, got)
+
+
+@pytest.mark.parametrize("p", ["s", "d", "c", "z"])
+def test_offsets_and_special_scalars(hc, handle, orc, p):
+    M, N, K = 70, 45, 33
+    dt = DT[p]
+    A = _rand(orc, p, M * K + 5, 1)
+    B = _rand(orc, p, K * N + 7, 2)
+    C = _rand(orc, p, M * N + 3, 3)
+    # offsets
+    got = _run(hc, handle, p, COL, N_, N_, M, N, K, 1.0, A, M, B, K, 0.5, C, M,
+               aoff=5, boff=7, coff=3)
+    _check_against_exact(orc, p, COL, N_, N_, M, N, K, 1.0, A[5:], M, B[7:], K,
+                         0.5, C[3:], M, got[3:])
+    assert bitwise_equal(got[:3], C[:3])
+    # beta == 0: NaN in C must not propagate
+    Cn = np.full(M * N, np.nan, dt)
+    got = _run(hc, handle, p, COL, N_, N_, M, N, K, 1.0, A, M, B, K, 0.0, Cn, M)
+    assert np.all(np.isfinite(got))
+    # alpha == 0: C = beta*C exactly (A, B not read)
+    got = _run(hc, handle, p, COL, N_, N_, M, N, K, 0.0, A, M, B, K, 2.0,
+               C[:M * N], M)
+    assert bitwise_equal(got, (C[:M * N] * dt(2)).astype(dt))
+    # K == 0: C = beta*C
+    got = _run(hc, handle, p, COL, N_, N_, M, N, 0, 1.0, A, M, B, 1, 0.5,
+               C[:M * N], M)
+    assert bitwise_equal(got, (C[:M * N] * dt(0.5)).astype(dt))
+    # M == 0 / N == 0: quick return, C untouched
+    got = _run(hc, handle, p, COL, N_, N_, 0, N, K, 1.0, A, 1, B, K, 0.5,
+               C[:M * N], 1)
+    assert bitwise_equal(got, C[:M * N])
+
+
+def test_sgemm_config1_rowmajor_1024(hc, handle, orc):
+    """Row-major NN 1024^3, A and B uniform[-1,1] from seed 0, C = 0,
+    alpha = 1, beta = 0 (the plain-SGEMM configuration)."""
+    M = N = K = 1024
+    A = orc.uniform_f32(M * K, 0, 0)
+    B = orc.uniform_f32(K * N, 0, M * K)
+    C = np.zeros(M * N, np.float32)
+    got = _run(hc, handle, "s", ROW, N_, N_, M, N, K, 1.0, A, K, B, N, 0.0, C,
+               N)
+    _check_against_exact(orc, "s", ROW, N_, N_, M, N, K, 1.0, A, K, B, N, 0.0,
+                         C, N, got, nsamp=4096)
+    # relative L2 (test/include/helper_functions.h:10-53, threshold 1e-5)
+    rows = np.arange(0, M, 7)
+    ref = orc.sgemm_ref64(ROW, N_, N_, 64, N, K, 1.0, A, K, B, N, 0.0, C, N)
+    g64 = got[:64 * N].astype(np.float64)
+    assert np.linalg.norm(g64 - ref) / np.linalg.norm(ref) <= 1e-5
+    assert rows.size
+
+
+@pytest.mark.parametrize("order", [COL, ROW])
+def test_hgemm_fp32_accumulate(hc, handle, orc, order):
+    api = hc.api
+    M, N, K = 131, 77, 300
+    rng = np.random.default_rng(5)
+    A = rng.uniform(-1, 1, M * K).astype(np.float16)
+    B = rng.uniform(-1, 1, K * N).astype(np.float16)
+    C = rng.uniform(-1, 1, M * N).astype(np.float16)
+    for ta in (N_, T_):
+        for tb in (N_, T_):
+            lda = (M if ta == N_ else K) if order == COL else (K if ta == N_ else M)
+            ldb = (K if tb == N_ else N) if order == COL else (N if tb == N_ else K)
+            ldc = M if order == COL else N
+            dC = to_dev(C.view(np.uint16))
+            st = api.hcblas_hgemm(handle, order, ta, tb, M, N, K, 1.0,
+                                  to_dev(A.view(np.uint16)), lda,
+                                  to_dev(B.view(np.uint16)), ldb, 0.5, dC, ldc,
+                                  0, 0, 0)
+            assert st == 0
+            got = to_host(dC).view(np.uint16).view(np.float16).astype(np.float64)
+            ref = orc.hgemm(order, ta, tb, M, N, K, 1.0, A, lda, B, ldb, 0.5, C,
+                            ldc).astype(np.float64)
+            # fp32 accumulation then one rounding to half: within one half ulp
+            # of the exactly rounded value plus the fp32 accumulation error.
+            ulp = np.spacing(np.abs(ref).astype(np.float16)).astype(np.float64)
+            assert np.all(np.abs(got - ref) <= ulp + K * 2 ** -23 * 2), (ta, tb)
+
+
+def test_hgemm_c_api(hc, handle):
+    api = hc.api
+    M, N, K = 64, 32, 48
+    A = np.full(M * K, hc.half_bits(0.5), np.uint16)
+    B = np.full(K * N, hc.half_bits(2.0), np.uint16)
+    C = np.full(M * N, hc.half_bits(1.0), np.uint16)
+    dC = to_dev(C)
+    st = api.hcblasHgemm(handle, N_, N_, M, N, K, 1.0, to_dev(A), M, to_dev(B),
+                         K, 2.0, dC, M)
+    assert st == 0
+    got = to_host(dC).view(np.uint16).view(np.float16)
+    assert np.all(got == K + 2)
+
+
+@pytest.mark.parametrize("p", ["s", "d", "c", "z"])
+def test_pointer_array_batched(hc, handle, orc, torch_cuda, p):
+    """test-blas.cpp:1922-2060: device array of device pointers, batch 32."""
+    torch = torch_cuda
+    api = hc.api
+    M, N, K, b = 123, 78, 23, 32
+    As = [_rand(orc, p, M * K, 10 + i) for i in range(b)]
+    Bs = [_rand(orc, p, K * N, 100 + i) for i in range(b)]
+    Cs = [_rand(orc, p, M * N, 1000 + i) for i in range(b)]
+    dA = [to_dev(a) for a in As]
+    dB = [to_dev(x) for x in Bs]
+    dC = [to_dev(c) for c in Cs]
+    pa = torch.tensor([t.data_ptr() for t in dA], dtype=torch.int64, device="cuda")
+    pb = torch.tensor([t.data_ptr() for t in dB], dtype=torch.int64, device="cuda")
+    pc = torch.tensor([t.data_ptr() for t in dC], dtype=torch.int64, device="cuda")
+    alpha = 1.0 if p in "sd" else 1 - 0.5j
+    beta = 1.0 if p in "sd" else 0.5 + 0j
+    fn = getattr(api, f"hcblas{p.upper()}gemmBatched")
+    st = fn(handle, N_, T_, M, N, K, alpha, pa, M, pb, N, beta, pc, M, b)
+    assert st == 0
+    for i in range(b):
+        got = to_host(dC[i])
+        _check_against_exact(orc, p, COL, N_, T_, M, N, K, alpha, As[i], M,
+                             Bs[i], N, beta, Cs[i], M, got, nsamp=128)
+
+
+@pytest.mark.parametrize("p", ["s", "d", "c", "z"])
+def test_strided_batched_row_major(hc, handle, orc, p):
+    """Row-major strided batches (the reference silently skipped row-major
+    batched GEMM, sgemm_array_wrapper.cpp:113-151)."""
+    api = hc.api
+    M, N, K, b = 40, 36, 50, 5
+    dtc = {"s": 0, "d": 1, "c": 2, "z": 3}[p]
+    sA, sB, sC = M * K + 4, K * N + 8, M * N + 12
+    A = _rand(orc, p, sA * b, 1)
+    B = _rand(orc, p, sB * b, 2)
+    C = _rand(orc, p, sC * b, 3)
+    dC = to_dev(C)
+    alpha = 1.25 if p in "sd" else 1.25 + 1j
+    st = api.hcblas_gemm_strided_batched(handle, dtc, ROW, T_, N_, M, N, K,
+                                         alpha, to_dev(A), M, 2, sA, to_dev(B),
+                                         N, 4, sB, 0.0, dC, N, 6, sC, b)
+    assert st == 0
+    got = to_host(dC)
+    for i in range(b):
+        _check_against_exact(orc, p, ROW, T_, N_, M, N, K, alpha,
+                             A[i * sA + 2:], M, B[i * sB + 4:], N, 0.0,
+                             C[i * sC + 6:], N, got[i * sC + 6:], nsamp=128)
+
+
+def test_bf16_strided_batched_fp32_out(hc, handle, torch_cuda):
+    """bf16 inputs, fp32 accumulation and fp32 output (the differentiation
+    configuration's forward shape, reduced batch)."""
+    torch = torch_cuda
+    api = hc.api
+    b, M, N, K = 4, 256, 1024, 512
+    g = torch.Generator(device="cuda").manual_seed(3)
+    A = (torch.rand(b, M, K, generator=g, device="cuda") * 2 - 1).bfloat16()
+    B = (torch.rand(b, K, N, generator=g, device="cuda") * 2 - 1).bfloat16()
+    C = torch.empty(b, M, N, device="cuda", dtype=torch.float32)
+    st = api.hcblas_gemm_strided_batched(handle, 5, ROW, N_, N_, M, N, K, 1.0,
+                                         A, K, 0, M * K, B, N, 0, K * N, 0.0,
+                                         C, N, 0, M * N, b)
+    assert st == 0
+    torch.cuda.synchronize()
+    ref = torch.bmm(A.double(), B.double().contiguous())
+    err = (C.double() - ref).abs().max().item()
+    assert err <= K * 2 ** -23 * 4, err
