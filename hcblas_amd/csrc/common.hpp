@@ -18,8 +18,13 @@ namespace hcb {
 
 constexpr uint32_t kHandleMagic = 0x48434d49u;  // "HCMI"
 
-// Reduction scratch: one double per workgroup partial, plus the final value.
-constexpr int kMaxPartials = 8192;
+// Reduction scratch (dot / asum): at most kMaxReduceBlocks workgroup
+// partials, reduced in groups of kReduceGroup, then across groups.
+constexpr int kMaxReduceBlocks = 1 << 21;
+constexpr int kReduceGroup = 256;
+constexpr int kMaxReduceGroups = kMaxReduceBlocks / kReduceGroup;
+constexpr size_t kMaxPartials = (size_t)kMaxReduceBlocks + kMaxReduceGroups;
+constexpr size_t kTicketCount = 1 + kMaxReduceGroups;
 
 struct Workspace {
   void *ptr = nullptr;
@@ -34,7 +39,7 @@ struct hcblasContext {
   int num_cus = 256;
   hipStream_t stream = nullptr;
   int math_mode = HCBLAS_DEFAULT_MATH;
-  int ozaki_slices = 7;
+  int ozaki_slices = 8;
 
   // dot / asum: per-workgroup partials, a ticket counter that the last
   // workgroup resets to zero, the device result and a pinned host mirror.
@@ -49,8 +54,12 @@ struct hcblasContext {
   hcb::Workspace gemv_partials;
   hcb::Workspace gemv_tickets;
 
-  // Packing / slicing buffers of the complex, split and Ozaki GEMM paths.
+  // Real embedding of complex GEMM operands.
   hcb::Workspace gemm_ws;
+  // BF16 planes / INT8 slices and exponents of the split GEMM paths (kept
+  // apart from gemm_ws: a split complex GEMM reads the embedding while it
+  // writes the planes).
+  hcb::Workspace split_ws;
 };
 
 namespace hcb {

@@ -23,22 +23,10 @@
 // folded into the A embedding (netlib folds it into B: temp = alpha*B(l,j)).
 #include <cstring>
 
+#include "gemm_common.hpp"
 #include "gemm_launch.hpp"
 
 namespace hcb {
-
-// Canonical column-major problem.
-struct Canon {
-  int M, N, K;
-  bool a_k, b_k;       // A' k-contiguous, B' k-contiguous
-  bool conj_a, conj_b; // complex conjugation of A', B'
-  const void *A;
-  int64_t lda, aOff, strideA;
-  const void *const *Aptrs;
-  const void *B;
-  int64_t ldb, bOff, strideB;
-  const void *const *Bptrs;
-};
 
 namespace {
 
@@ -146,6 +134,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void scale_c_complex_kernel(
     int M, int N, T *C, int64_t ldc, int64_t strideC, void *const *Cptrs,
     int64_t cOff, T br, T bi) {
+#pragma clang fp contract(off)
   T *Cb = Cptrs ? static_cast<T *>(Cptrs[blockIdx.z]) + 2 * cOff
                 : C + 2 * (cOff + (int64_t)blockIdx.z * strideC);
   const int64_t total = (int64_t)M * N;
@@ -174,6 +163,7 @@ __global__ __launch_bounds__(256) void complex_embed_a_kernel(
     int M, int K, const T *A, int64_t lda, int64_t aOff, int64_t strideA,
     const void *const *Aptrs, bool a_k, bool conj, T ar, T ai, T *Ar,
     int64_t ldr, int64_t strideR) {
+#pragma clang fp contract(off)
   __shared__ T tre[32][33], tim[32][33];
   const int z = blockIdx.z;
   const T *Ab = Aptrs ? static_cast<const T *>(Aptrs[z]) + 2 * aOff
@@ -440,7 +430,17 @@ hcblasStatus_t gemm_complex(hcblasHandle_t h, const Canon &c, T ar, T ai,
     r.strideB = 2 * c.strideB;
     r.Bptrs = c.Bptrs;
   }
-  // Pointer-array C: the kernel adds cOff in real elements.
+  // Pointer-array C: the kernel adds cOff in real elements. The embedded
+  // real problem follows the handle's math mode (split CGEMM, Ozaki ZGEMM).
+  if constexpr (E == Elem::F32) {
+    if (h->math_mode & HCBLAS_SPLIT_BF16_MATH)
+      return gemm_split_bf16(h, r, 1.f, (float)br, static_cast<float *>(C),
+                             2 * ldc, 2 * cOff, 2 * strideC, Cptrs, batch);
+  } else {
+    if (h->math_mode & HCBLAS_OZAKI_INT8_MATH)
+      return gemm_ozaki_int8(h, r, 1.0, (double)br, static_cast<double *>(C),
+                             2 * ldc, 2 * cOff, 2 * strideC, Cptrs, batch);
+  }
   return gemm_real<E, T>(h, r, 1.0, (double)br, C, 2 * ldc, 2 * cOff,
                          2 * strideC, Cptrs, batch);
 }
@@ -462,18 +462,6 @@ template hcblasStatus_t gemm_real<Elem::F64, double>(hcblasHandle_t,
 // Entry points
 // ---------------------------------------------------------------------------
 using namespace hcb;
-
-namespace hcb {
-// Split / Ozaki drivers live in their own translation units.
-hcblasStatus_t gemm_split_bf16(hcblasHandle_t h, const Canon &c, float alpha,
-                               float beta, float *C, int64_t ldc, int64_t cOff,
-                               int64_t strideC, void *const *Cptrs,
-                               int64_t batch);
-hcblasStatus_t gemm_ozaki_int8(hcblasHandle_t h, const Canon &c, double alpha,
-                               double beta, double *C, int64_t ldc,
-                               int64_t cOff, int64_t strideC,
-                               void *const *Cptrs, int64_t batch);
-}  // namespace hcb
 
 namespace {
 

@@ -23,7 +23,8 @@
 //     groups (lane/16) hold four disjoint k sub-ranges, and the same mapping is
 //     used for A and B, so the MFMA sums exactly the products of the chunk.
 //   - MFMA output layout (16x16, four values per lane): value r of lane l is
-//     C(4*(l/16) + r, l%16).
+//     C(4*(l/16) + r, l%16) for the FP32/INT32-accumulating forms and
+//     C(l/16 + 4*r, l%16) for v_mfma_f64_16x16x4f64 (out_row below).
 //   - tile order: consecutive workgroup ids are spread over the 8 XCDs by the
 //     dispatcher, so ids are remapped to give each XCD a contiguous slice of
 //     a GROUP_M-grouped tile order (A and B panels stay in that XCD's L2).
@@ -175,7 +176,29 @@ struct TileLoader {
   static constexpr int NV = ROWS * BK / VEC;  // 16-byte vectors per tile
   static constexpr int PER_THREAD = (NV + 255) / 256;
   static constexpr int ROWB = BK * (int)sizeof(T) + 16;  // LDS row stride
+  // Transposing (r-contiguous) tiles: each group of 64 vectors covers
+  // 8 r-vectors x 8 k, so a wave reads 128-byte runs of global memory and its
+  // element-wise LDS stores spread over the banks (2-way instead of 8-way
+  // conflicts for 32 r-vectors of one k).
+  static constexpr bool BLOCKED = !KCONT && (ROWS / VEC) % 8 == 0 &&
+                                  BK % 8 == 0 && NV % 64 == 0;
   uint4 reg[PER_THREAD];
+
+  // (r, k) of vector v of the tile (r, k are element indices).
+  __device__ __forceinline__ static void vec_pos(int v, int &r, int &k) {
+    if (KCONT) {
+      r = v / (BK / VEC);
+      k = (v % (BK / VEC)) * VEC;
+    } else if (BLOCKED) {
+      constexpr int NRB = ROWS / VEC / 8;
+      const int w = v >> 6, l = v & 63;
+      r = ((w % NRB) * 8 + (l & 7)) * VEC;
+      k = (w / NRB) * 8 + (l >> 3);
+    } else {
+      k = v / (ROWS / VEC);
+      r = (v % (ROWS / VEC)) * VEC;
+    }
+  }
 
   __device__ __forceinline__ void load(const T *__restrict__ P, int64_t ld,
                                        int r0, int k0, int R, int K) {
@@ -184,13 +207,7 @@ struct TileLoader {
       const int v = threadIdx.x + i * 256;
       if (NV % 256 != 0 && v >= NV) continue;
       int r, k;
-      if (KCONT) {
-        r = v / (BK / VEC);
-        k = (v % (BK / VEC)) * VEC;
-      } else {
-        k = v / (ROWS / VEC);
-        r = (v % (ROWS / VEC)) * VEC;
-      }
+      vec_pos(v, r, k);
       const int gr = r0 + r, gk = k0 + k;
       if (KCONT) {
         const T *src = P + (int64_t)gr * ld + gk;
@@ -223,14 +240,12 @@ struct TileLoader {
     for (int i = 0; i < PER_THREAD; ++i) {
       const int v = threadIdx.x + i * 256;
       if (NV % 256 != 0 && v >= NV) continue;
+      int r, k;
+      vec_pos(v, r, k);
       if (KCONT) {
-        const int r = v / (BK / VEC);
-        const int k = (v % (BK / VEC)) * VEC;
         *reinterpret_cast<uint4 *>(lds + r * ROWB + k * (int)sizeof(T)) =
             reg[i];
       } else {
-        const int k = v / (ROWS / VEC);
-        const int r = (v % (ROWS / VEC)) * VEC;
         T t[VEC];
         __builtin_memcpy(t, &reg[i], 16);
 #pragma unroll
@@ -256,6 +271,15 @@ __device__ __forceinline__ void tile_coords(int pid, int tm, int tn,
   const int in_group = pid % width;
   tile_m = first_m + in_group % gsize;
   tile_n = in_group / gsize;
+}
+
+// Row (within a 16x16 MFMA block) of accumulator value r held by lane group g.
+template <Elem E>
+__device__ __forceinline__ int out_row(int g, int r) {
+  if constexpr (E == Elem::F64)
+    return g + 4 * r;
+  else
+    return 4 * g + r;
 }
 
 // Output conversion.
@@ -422,10 +446,10 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmParams p) {
     for (int j = 0; j < FN; ++j) {
       const int col = n0 + wn * WTN + j * 16 + lrow;
       if (col >= p.N) continue;
-      const int rowb = m0 + wm * WTM + i * 16 + lgrp * 4;
+      const int rowb = m0 + wm * WTM + i * 16;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = rowb + r;
+        const int row = rowb + out_row<E>(lgrp, r);
         if (row >= p.M) continue;
         OutT *dst = C + (int64_t)col * p.ldc + row;
         Acc v = alpha * (Acc)acc[i][j][r];

@@ -1,0 +1,205 @@
+// Plane-group MFMA GEMM: the common core of the split-precision paths.
+//
+// Operands arrive pre-packed, k-contiguous, as S planes per row:
+//   Ap[m][q][0..Kp)  (q = 0..S-1),   Bp[n][q][0..Kp)
+// A "group" g multiplies the window of np_g consecutive A planes starting at
+// plane a0_g with the window of np_g consecutive B planes starting at b0_g:
+//     G_g(m,n) = sum_{j<np_g} sum_k Ap[m][a0_g+j][k] * Bp[n][b0_g+j][k]
+// which is one long-K GEMM (K = np_g*Kp) because the windows are contiguous.
+// Groups collect the products of equal magnitude:
+//   - BF16 split SGEMM: A planes (hi, mid, lo), B planes (lo, mid, hi);
+//     groups {A[0..3) x B[0..3)} = hi*lo + mid*mid + lo*hi,
+//            {A[0..2) x B[1..3)} = hi*mid + mid*hi,
+//            {A[0..1) x B[2..3)} = hi*hi,
+//     accumulated smallest first into one FP32 MFMA accumulator.
+//   - Ozaki INT8 DGEMM: A planes = slices 1..s, B planes = slices s..1; group
+//     d (t+u = d) is A[0..d-1) x B[s-d+1..s), summed exactly in INT32 and
+//     added to an FP64 accumulator with weight 2^(-7(d-2)); the epilogue
+//     applies the per-row / per-column power-of-two scales.
+#pragma once
+
+#include <climits>
+
+#include "gemm_kernel.hpp"
+
+namespace hcb {
+
+constexpr int kMaxGroups = 16;
+
+struct PlaneParams {
+  int M, N;
+  int64_t Kp;       // padded K per plane (multiple of BK)
+  int S;            // planes per row
+  const void *Ap;   // M rows of S*Kp elements
+  const void *Bp;   // N rows of S*Kp elements
+  int ngroups;
+  int a0[kMaxGroups], b0[kMaxGroups], np[kMaxGroups];
+  double weight[kMaxGroups];  // INT8 only: scale of group g
+  const int *row_exp;         // INT8 only: per-row exponent of A (M)
+  const int *col_exp;         // INT8 only: per-column exponent of B (N)
+  int exp_bias;               // INT8 only: C scale = 2^(row+col+bias)
+  void *C;                    // column-major M x N (float or double)
+  int64_t ldc;
+  double alpha, beta;
+  int tiles_m, tiles_n;
+};
+
+template <Elem E, typename OutT, int BM, int BN, int BK, int WM, int WN>
+__global__ __launch_bounds__(256) void gemm_planes_kernel(PlaneParams p) {
+  using Tr = ElemT<E>;
+  using T = typename Tr::T;
+  using AccV = typename Tr::AccV;
+  constexpr bool INT = E == Elem::I8;
+  constexpr int VEC = Tr::VEC;
+  constexpr int CK = 4 * VEC;
+  static_assert(Tr::STEPS == 1, "packed types only");
+  static_assert(BK % CK == 0, "BK");
+  constexpr int WTM = BM / WM, WTN = BN / WN;
+  constexpr int FM = WTM / 16, FN = WTN / 16;
+  using LA = TileLoader<T, BM, BK, true, true>;
+  using LB = TileLoader<T, BN, BK, true, true>;
+  constexpr int ROWB = LA::ROWB;
+  constexpr int A_BYTES = BM * ROWB, B_BYTES = BN * ROWB;
+  __shared__ __attribute__((aligned(16))) char lds[2 * (A_BYTES + B_BYTES)];
+
+  int tile_m, tile_n;
+  tile_coords(blockIdx.x, p.tiles_m, p.tiles_n, tile_m, tile_n);
+  const int m0 = tile_m * BM, n0 = tile_n * BN;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wm = wave / WN, wn = wave % WN;
+  const int lrow = lane & 15, lgrp = lane >> 4;
+  const int64_t ld = (int64_t)p.S * p.Kp;
+  const T *Ap = static_cast<const T *>(p.Ap);
+  const T *Bp = static_cast<const T *>(p.Bp);
+
+  AccV acc[FM][FN];
+  double facc[INT ? FM : 1][INT ? FN : 1][INT ? 4 : 1];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = AccV{};
+  if constexpr (INT) {
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) facc[i][j][r] = 0.0;
+  }
+
+  const int kt_per_plane = (int)(p.Kp / BK);
+  // Flat tile sequence over all groups.
+  int total = 0;
+  for (int g = 0; g < p.ngroups; ++g) total += p.np[g] * kt_per_plane;
+
+  LA la;
+  LB lb;
+  // (g, kt) cursors for the tile being loaded.
+  int lg = 0, lkt = 0;
+  auto load_tile = [&](int g, int kt) {
+    const int64_t ka = (int64_t)p.a0[g] * p.Kp + (int64_t)kt * BK;
+    const int64_t kb = (int64_t)p.b0[g] * p.Kp + (int64_t)kt * BK;
+    la.load(Ap + ka, ld, m0, 0, p.M, BK);
+    lb.load(Bp + kb, ld, n0, 0, p.N, BK);
+  };
+  load_tile(0, 0);
+  la.store(lds);
+  lb.store(lds + A_BYTES);
+  __syncthreads();
+  int cg = 0, ckt = 0;  // group / tile being computed
+  lg = 0;
+  lkt = 0;
+
+  for (int t = 0; t < total; ++t) {
+    char *cur = lds + (t & 1) * (A_BYTES + B_BYTES);
+    char *nxt = lds + ((t + 1) & 1) * (A_BYTES + B_BYTES);
+    const bool more = t + 1 < total;
+    if (more) {
+      if (++lkt == p.np[lg] * kt_per_plane) {
+        lkt = 0;
+        ++lg;
+      }
+      load_tile(lg, lkt);
+    }
+    const char *As = cur;
+    const char *Bs = cur + A_BYTES;
+#pragma unroll
+    for (int c = 0; c < BK / CK; ++c) {
+      uint4 af[FM], bf[FN];
+      const int kbyte = (c * CK + lgrp * VEC) * (int)sizeof(T);
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+        af[i] = *reinterpret_cast<const uint4 *>(
+            As + (wm * WTM + i * 16 + lrow) * ROWB + kbyte);
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+        bf[j] = *reinterpret_cast<const uint4 *>(
+            Bs + (wn * WTN + j * 16 + lrow) * ROWB + kbyte);
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) mfma_step<E, 0>(af[i], bf[j], acc[i][j]);
+    }
+    if (more) {
+      la.store(nxt);
+      lb.store(nxt + A_BYTES);
+    }
+    // End of a group: INT32 group sums are exact; fold them into FP64.
+    if (++ckt == p.np[cg] * kt_per_plane) {
+      if constexpr (INT) {
+        const double w = p.weight[cg];
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+              facc[i][j][r] = __builtin_fma((double)acc[i][j][r], w,
+                                            facc[i][j][r]);
+            acc[i][j] = AccV{};
+          }
+      }
+      ckt = 0;
+      ++cg;
+    }
+    __syncthreads();
+  }
+
+  OutT *C = static_cast<OutT *>(p.C);
+#pragma unroll
+  for (int i = 0; i < FM; ++i) {
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int col = n0 + wn * WTN + j * 16 + lrow;
+      if (col >= p.N) continue;
+      const int rowb = m0 + wm * WTM + i * 16 + lgrp * 4;
+      int ce = 0;
+      if constexpr (INT) ce = p.col_exp[col];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = rowb + r;
+        if (row >= p.M) continue;
+        OutT *dst = C + (int64_t)col * p.ldc + row;
+        if constexpr (INT) {
+          // An all-zero row / column has no exponent (INT_MIN) and a zero
+          // accumulator.
+          const int re = p.row_exp[row];
+          const double v =
+              (re == INT_MIN || ce == INT_MIN)
+                  ? 0.0
+                  : __builtin_ldexp(facc[i][j][r], re + ce + p.exp_bias);
+          double o = p.alpha * v;
+          if (p.beta != 0.0) o = __builtin_fma(p.beta, (double)*dst, o);
+          *dst = (OutT)o;
+        } else {
+          const float a = (float)p.alpha, b = (float)p.beta;
+          float o = a * (float)acc[i][j][r];
+          if (b != 0.f) o = __builtin_fmaf(b, (float)*dst, o);
+          *dst = (OutT)o;
+        }
+      }
+    }
+  }
+}
+
+}  // namespace hcb

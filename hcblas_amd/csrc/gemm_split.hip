@@ -1,16 +1,362 @@
-// Split-precision GEMM drivers (BF16 split SGEMM, Ozaki INT8 DGEMM).
-#include "gemm_launch.hpp"
+// Split-precision GEMM (no counterpart in the reference):
+//   - SGEMM / CGEMM as three-way BF16 splits on BF16 MFMA with FP32
+//     accumulation (six products: hi*hi, hi*mid, mid*hi, hi*lo, mid*mid,
+//     lo*hi);
+//   - DGEMM / ZGEMM as Ozaki exact INT8 slices on INT8 MFMA.
+// Both pack the operands once into k-contiguous planes and run the
+// plane-group kernel of gemm_planes.hpp.
+//
+// BF16 split: x = hi + mid + lo + d with hi = bf16(x), mid = bf16(x - hi),
+// lo = bf16(x - hi - mid) (round-to-nearest-even; the residuals are exact in
+// FP32), so |d| <= 2^-24 |x|; the dropped products (mid*lo, lo*mid, lo*lo and
+// d terms) are below 2^-24 relative.
+//
+// Ozaki slicing (scheme I with power-of-two row / column scaling):
+//   e_m = exponent of max_k |A'(m,k)| (|A'(m,k)| < 2^e_m), v = A' * 2^(6-e_m)
+//   a_1 = rint(v), a_t = rint(128 * (previous residual)), t = 1..s,
+// every a_t in [-64, 64]; A' = 2^(e_m-6) * sum_t a_t 2^(-7(t-1)) + O(2^-7s).
+// Same for B' by columns. Products with t + u = d share the weight
+// 2^(-7(d-2)); their INT32 sums are exact while s*K*64*64 < 2^31, and K is
+// processed in chunks when it is larger. Only d <= s+1 is kept.
+// FP64 is used only for the exponent scan, the slicing and the recombination.
+#include <climits>
+#include <vector>
+
+#include "gemm_common.hpp"
+#include "gemm_planes.hpp"
 
 namespace hcb {
-struct Canon;
-hcblasStatus_t gemm_split_bf16(hcblasHandle_t, const Canon &, float, float,
-                               float *, int64_t, int64_t, int64_t,
-                               void *const *, int64_t) {
-  return HCBLAS_STATUS_INTERNAL_ERROR;
+namespace {
+
+__device__ __forceinline__ uint16_t f32_to_bf16_rne(float x) {
+  uint32_t u = __float_as_uint(x);
+  uint32_t r = u + 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(r >> 16);
 }
-hcblasStatus_t gemm_ozaki_int8(hcblasHandle_t, const Canon &, double, double,
-                               double *, int64_t, int64_t, int64_t,
-                               void *const *, int64_t) {
-  return HCBLAS_STATUS_INTERNAL_ERROR;
+__device__ __forceinline__ float bf16_to_f32(uint16_t b) {
+  return __uint_as_float((uint32_t)b << 16);
 }
+
+// Reads element (r, k) of an R x K operand view with the given orientation
+// (kcont: element at X[k + r*ld], else X[r + k*ld]).
+template <typename T>
+__device__ __forceinline__ T read_rk(const T *X, int64_t ld, bool kcont, int r,
+                                     int k) {
+  return kcont ? X[(int64_t)r * ld + k] : X[(int64_t)k * ld + r];
+}
+
+// 32 x 32 (r, k) tile staged through LDS: reads along the source's
+// contiguous dimension, hands each thread (r, k) pairs along k.
+template <typename T>
+struct TileStage {
+  T v[32][33];
+  __device__ void load(const T *X, int64_t ld, bool kcont, int R, int K,
+                       int r0, int k0) {
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int i = ty; i < 32; i += 8) {
+      const int r = kcont ? r0 + i : r0 + tx;
+      const int k = kcont ? k0 + tx : k0 + i;
+      T x = (r < R && k < K) ? read_rk(X, ld, kcont, r, k) : T(0);
+      if (kcont)
+        v[i][tx] = x;
+      else
+        v[tx][i] = x;
+    }
+    __syncthreads();
+  }
+};
+
+// BF16 three-way split into planes P[r][q][0..Kp), q = plane index
+// (hi, mid, lo), or (lo, mid, hi) when reverse. Covers k < Kp (zeros past K).
+__global__ __launch_bounds__(256) void split_bf16_kernel(
+    int R, int K, const float *X, int64_t ld, bool kcont, uint16_t *P,
+    int64_t Kp, bool reverse) {
+  __shared__ TileStage<float> st;
+  const int r0 = blockIdx.y * 32, k0 = blockIdx.x * 32;
+  st.load(X, ld, kcont, R, K, r0, k0);
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int i = ty; i < 32; i += 8) {
+    const int r = r0 + i, k = k0 + tx;
+    if (r >= R || k >= Kp) continue;
+    const float x = st.v[i][tx];
+    const uint16_t h = f32_to_bf16_rne(x);
+    const float r1 = x - bf16_to_f32(h);
+    const uint16_t m = f32_to_bf16_rne(r1);
+    const float r2 = r1 - bf16_to_f32(m);
+    const uint16_t l = f32_to_bf16_rne(r2);
+    uint16_t *row = P + (int64_t)r * 3 * Kp + k;
+    row[(reverse ? 2 : 0) * Kp] = h;
+    row[1 * Kp] = m;
+    row[(reverse ? 0 : 2) * Kp] = l;
+  }
+}
+
+// Smallest e with |x| < 2^e (x finite, nonzero): frexp gives x = f*2^e,
+// 0.5 <= |f| < 1.
+__device__ __forceinline__ int exp_bound(double x) {
+  int e;
+  (void)frexp(x, &e);
+  return e;
+}
+
+__global__ void fill_int_kernel(int *p, int n, int v) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// Row exponents: exps[r] = max over nonzero x of ilogb(x) + 1, so that
+// |x| < 2^exps[r]; rows without a nonzero keep INT_MIN.
+// k-contiguous rows: a workgroup reduces 1024 k of one row.
+__global__ __launch_bounds__(256) void row_exp_kcont_kernel(
+    int R, int K, const double *X, int64_t ld, int *exps) {
+  const int r = blockIdx.x;
+  const int kb = blockIdx.y * 1024;
+  int e = INT_MIN;
+  for (int k = kb + threadIdx.x; k < K && k < kb + 1024; k += 256) {
+    const double x = X[(int64_t)r * ld + k];
+    if (x != 0.0) e = max(e, exp_bound(x));
+  }
+  for (int off = 32; off >= 1; off >>= 1) e = max(e, __shfl_xor(e, off, 64));
+  __shared__ int we[4];
+  if ((threadIdx.x & 63) == 0) we[threadIdx.x >> 6] = e;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int m = max(max(we[0], we[1]), max(we[2], we[3]));
+    if (m != INT_MIN) atomicMax(&exps[r], m);
+  }
+}
+// r-contiguous rows: thread per row over a k range of 256.
+__global__ __launch_bounds__(256) void row_exp_rcont_kernel(
+    int R, int K, const double *X, int64_t ld, int *exps) {
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= R) return;
+  const int kb = blockIdx.y * 256;
+  int e = INT_MIN;
+  for (int k = kb; k < K && k < kb + 256; ++k) {
+    const double x = X[(int64_t)k * ld + r];
+    if (x != 0.0) e = max(e, exp_bound(x));
+  }
+  if (e != INT_MIN) atomicMax(&exps[r], e);
+}
+
+// INT8 slices into planes P[r][q][0..Kp) (q = t, or s-1-t when reverse).
+__global__ __launch_bounds__(256) void ozaki_slice_kernel(
+    int R, int K, const double *X, int64_t ld, bool kcont, const int *exps,
+    int s, int8_t *P, int64_t Kp, bool reverse) {
+  __shared__ TileStage<double> st;
+  const int r0 = blockIdx.y * 32, k0 = blockIdx.x * 32;
+  st.load(X, ld, kcont, R, K, r0, k0);
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int i = ty; i < 32; i += 8) {
+    const int r = r0 + i, k = k0 + tx;
+    if (r >= R || k >= Kp) continue;
+    const int e = exps[r];
+    double v = e == INT_MIN ? 0.0 : __builtin_ldexp(st.v[i][tx], 6 - e);
+    int8_t *row = P + (int64_t)r * s * Kp + k;
+    for (int t = 0; t < s; ++t) {
+      const double a = __builtin_rint(v);
+      v = (v - a) * 128.0;
+      row[(int64_t)(reverse ? s - 1 - t : t) * Kp] = (int8_t)(int)a;
+    }
+  }
+}
+
+template <Elem E, typename OutT, int BM, int BN, int BK>
+hcblasStatus_t launch_planes(hipStream_t stream, PlaneParams p) {
+  p.tiles_m = (int)cdiv(p.M, BM);
+  p.tiles_n = (int)cdiv(p.N, BN);
+  gemm_planes_kernel<E, OutT, BM, BN, BK, 2, 2>
+      <<<p.tiles_m * p.tiles_n, 256, 0, stream>>>(p);
+  return launch_status();
+}
+
+// Host view of the batch entries (pointer arrays are read back once).
+struct BatchView {
+  std::vector<const void *> A, B;
+  std::vector<void *> C;
+};
+
+hcblasStatus_t batch_pointers(hcblasHandle_t h, const Canon &c, void *C,
+                              int64_t strideC, void *const *Cptrs,
+                              int64_t batch, size_t esize, BatchView &v) {
+  v.A.resize(batch);
+  v.B.resize(batch);
+  v.C.resize(batch);
+  auto fetch = [&](const void *const *dev, std::vector<const void *> &out) {
+    return hipMemcpy(out.data(), dev, sizeof(void *) * batch,
+                     hipMemcpyDeviceToHost) == hipSuccess;
+  };
+  if (c.Aptrs) {
+    if (!fetch(c.Aptrs, v.A)) return HCBLAS_STATUS_EXECUTION_FAILED;
+  }
+  if (c.Bptrs) {
+    if (!fetch(c.Bptrs, v.B)) return HCBLAS_STATUS_EXECUTION_FAILED;
+  }
+  if (Cptrs) {
+    std::vector<const void *> tmp(batch);
+    if (!fetch(reinterpret_cast<const void *const *>(Cptrs), tmp))
+      return HCBLAS_STATUS_EXECUTION_FAILED;
+    for (int64_t i = 0; i < batch; ++i) v.C[i] = const_cast<void *>(tmp[i]);
+  }
+  for (int64_t i = 0; i < batch; ++i) {
+    if (!c.Aptrs)
+      v.A[i] = static_cast<const char *>(c.A) + i * c.strideA * esize;
+    if (!c.Bptrs)
+      v.B[i] = static_cast<const char *>(c.B) + i * c.strideB * esize;
+    if (!Cptrs) v.C[i] = static_cast<char *>(C) + i * strideC * esize;
+  }
+  (void)h;
+  return HCBLAS_STATUS_SUCCESS;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+hcblasStatus_t gemm_split_bf16(hcblasHandle_t h, const Canon &c, float alpha,
+                               float beta, float *C, int64_t ldc, int64_t cOff,
+                               int64_t strideC, void *const *Cptrs,
+                               int64_t batch) {
+  if (c.M == 0 || c.N == 0 || batch == 0) return HCBLAS_STATUS_SUCCESS;
+  if (alpha == 0.f || c.K == 0)
+    return gemm_real<Elem::F32, float>(h, c, alpha, beta, C, ldc, cOff,
+                                       strideC, Cptrs, batch);
+  constexpr int BK = 64;
+  const int64_t Kp = cdiv(c.K, BK) * BK;
+  const size_t a_bytes = sizeof(uint16_t) * 3 * (size_t)c.M * Kp;
+  const size_t b_bytes = sizeof(uint16_t) * 3 * (size_t)c.N * Kp;
+  if (ensure_workspace(h->split_ws, a_bytes + b_bytes) != hipSuccess)
+    return HCBLAS_STATUS_ALLOC_FAILED;
+  uint16_t *Ap = static_cast<uint16_t *>(h->split_ws.ptr);
+  uint16_t *Bp = Ap + a_bytes / sizeof(uint16_t);
+  BatchView bv;
+  hcblasStatus_t st =
+      batch_pointers(h, c, C, strideC, Cptrs, batch, sizeof(float), bv);
+  if (st != HCBLAS_STATUS_SUCCESS) return st;
+  for (int64_t z = 0; z < batch; ++z) {
+    const float *A = static_cast<const float *>(bv.A[z]) + c.aOff;
+    const float *B = static_cast<const float *>(bv.B[z]) + c.bOff;
+    float *Cz = static_cast<float *>(bv.C[z]) + cOff;
+    dim3 ga((unsigned)cdiv(Kp, 32), (unsigned)cdiv(c.M, 32));
+    split_bf16_kernel<<<ga, 256, 0, h->stream>>>(c.M, c.K, A, c.lda, c.a_k, Ap,
+                                                 Kp, false);
+    dim3 gb((unsigned)cdiv(Kp, 32), (unsigned)cdiv(c.N, 32));
+    split_bf16_kernel<<<gb, 256, 0, h->stream>>>(c.N, c.K, B, c.ldb, c.b_k, Bp,
+                                                 Kp, true);
+    st = launch_status();
+    if (st != HCBLAS_STATUS_SUCCESS) return st;
+    PlaneParams p{};
+    p.M = c.M;
+    p.N = c.N;
+    p.Kp = Kp;
+    p.S = 3;
+    p.Ap = Ap;
+    p.Bp = Bp;
+    // smallest group first: hi*lo + mid*mid + lo*hi, hi*mid + mid*hi, hi*hi
+    p.ngroups = 3;
+    p.a0[0] = 0, p.b0[0] = 0, p.np[0] = 3;
+    p.a0[1] = 0, p.b0[1] = 1, p.np[1] = 2;
+    p.a0[2] = 0, p.b0[2] = 2, p.np[2] = 1;
+    p.C = Cz;
+    p.ldc = ldc;
+    p.alpha = alpha;
+    p.beta = beta;
+    st = launch_planes<Elem::BF16, float, 128, 128, BK>(h->stream, p);
+    if (st != HCBLAS_STATUS_SUCCESS) return st;
+  }
+  return HCBLAS_STATUS_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
+hcblasStatus_t gemm_ozaki_int8(hcblasHandle_t h, const Canon &c, double alpha,
+                               double beta, double *C, int64_t ldc,
+                               int64_t cOff, int64_t strideC,
+                               void *const *Cptrs, int64_t batch) {
+  if (c.M == 0 || c.N == 0 || batch == 0) return HCBLAS_STATUS_SUCCESS;
+  if (alpha == 0.0 || c.K == 0)
+    return gemm_real<Elem::F64, double>(h, c, alpha, beta, C, ldc, cOff,
+                                        strideC, Cptrs, batch);
+  const int s = h->ozaki_slices;
+  constexpr int BK = 128;
+  // INT32 exactness: s * Kc * 64 * 64 < 2^31.
+  const int64_t kmax = ((int64_t)1 << 31) / (4096 * (int64_t)s) / BK * BK;
+  const int64_t kchunk = std::min<int64_t>(cdiv(c.K, BK) * BK, kmax);
+  const int64_t Kp = kchunk;
+  const size_t a_bytes = (size_t)s * c.M * Kp;
+  const size_t b_bytes = (size_t)s * c.N * Kp;
+  const size_t e_bytes = sizeof(int) * ((size_t)c.M + c.N);
+  if (ensure_workspace(h->split_ws, a_bytes + b_bytes + e_bytes + 256) !=
+      hipSuccess)
+    return HCBLAS_STATUS_ALLOC_FAILED;
+  int8_t *Ap = static_cast<int8_t *>(h->split_ws.ptr);
+  int8_t *Bp = Ap + a_bytes;
+  int *exps = reinterpret_cast<int *>(
+      (reinterpret_cast<uintptr_t>(Bp + b_bytes) + 255) & ~uintptr_t(255));
+  int *ea = exps, *eb = exps + c.M;
+  BatchView bv;
+  hcblasStatus_t st =
+      batch_pointers(h, c, C, strideC, Cptrs, batch, sizeof(double), bv);
+  if (st != HCBLAS_STATUS_SUCCESS) return st;
+  for (int64_t z = 0; z < batch; ++z) {
+    double *Cz = static_cast<double *>(bv.C[z]) + cOff;
+    for (int64_t k0 = 0; k0 < c.K; k0 += kchunk) {
+      const int kc = (int)std::min<int64_t>(kchunk, c.K - k0);
+      // Operand views of this K chunk.
+      const double *A = static_cast<const double *>(bv.A[z]) + c.aOff +
+                        (c.a_k ? k0 : k0 * c.lda);
+      const double *B = static_cast<const double *>(bv.B[z]) + c.bOff +
+                        (c.b_k ? k0 : k0 * c.ldb);
+      fill_int_kernel<<<(unsigned)cdiv(c.M + c.N, 256), 256, 0, h->stream>>>(
+          exps, c.M + c.N, INT_MIN);
+      if (c.a_k)
+        row_exp_kcont_kernel<<<dim3(c.M, (unsigned)cdiv(kc, 1024)), 256, 0,
+                               h->stream>>>(c.M, kc, A, c.lda, ea);
+      else
+        row_exp_rcont_kernel<<<dim3((unsigned)cdiv(c.M, 256),
+                                    (unsigned)cdiv(kc, 256)),
+                               256, 0, h->stream>>>(c.M, kc, A, c.lda, ea);
+      if (c.b_k)
+        row_exp_kcont_kernel<<<dim3(c.N, (unsigned)cdiv(kc, 1024)), 256, 0,
+                               h->stream>>>(c.N, kc, B, c.ldb, eb);
+      else
+        row_exp_rcont_kernel<<<dim3((unsigned)cdiv(c.N, 256),
+                                    (unsigned)cdiv(kc, 256)),
+                               256, 0, h->stream>>>(c.N, kc, B, c.ldb, eb);
+      dim3 ga((unsigned)cdiv(Kp, 32), (unsigned)cdiv(c.M, 32));
+      ozaki_slice_kernel<<<ga, 256, 0, h->stream>>>(c.M, kc, A, c.lda, c.a_k,
+                                                    ea, s, Ap, Kp, false);
+      dim3 gb((unsigned)cdiv(Kp, 32), (unsigned)cdiv(c.N, 32));
+      ozaki_slice_kernel<<<gb, 256, 0, h->stream>>>(c.N, kc, B, c.ldb, c.b_k,
+                                                    eb, s, Bp, Kp, true);
+      st = launch_status();
+      if (st != HCBLAS_STATUS_SUCCESS) return st;
+      PlaneParams p{};
+      p.M = c.M;
+      p.N = c.N;
+      p.Kp = Kp;
+      p.S = s;
+      p.Ap = Ap;
+      p.Bp = Bp;
+      // group d = t + u (smallest weight first): A[0..d-1) x B[s-d+1..s)
+      p.ngroups = s;
+      for (int g = 0; g < s; ++g) {
+        const int d = s + 1 - g;
+        p.a0[g] = 0;
+        p.b0[g] = s - d + 1;
+        p.np[g] = d - 1;
+        p.weight[g] = __builtin_ldexp(1.0, -7 * (d - 2));
+      }
+      p.row_exp = ea;
+      p.col_exp = eb;
+      p.exp_bias = -12;
+      p.C = Cz;
+      p.ldc = ldc;
+      p.alpha = alpha;
+      p.beta = k0 == 0 ? beta : 1.0;
+      st = launch_planes<Elem::I8, double, 128, 128, BK>(h->stream, p);
+      if (st != HCBLAS_STATUS_SUCCESS) return st;
+    }
+  }
+  return HCBLAS_STATUS_SUCCESS;
+}
+
 }  // namespace hcb

@@ -4,6 +4,7 @@
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <vector>
 
 #include "common.hpp"
 
@@ -60,11 +61,13 @@ hcblasStatus_t hcblasCreate(hcblasHandle_t *handle) {
 
   bool ok = hipMalloc(&h->d_partials, sizeof(double) * kMaxPartials) ==
                 hipSuccess &&
-            hipMalloc(&h->d_ticket, 256) == hipSuccess &&
+            hipMalloc(&h->d_ticket, sizeof(unsigned int) * kTicketCount) ==
+                hipSuccess &&
             hipMalloc(&h->d_result, 64) == hipSuccess &&
             hipHostMalloc(&h->h_result, 64, hipHostMallocDefault) ==
                 hipSuccess &&
-            hipMemset(h->d_ticket, 0, 256) == hipSuccess &&
+            hipMemset(h->d_ticket, 0, sizeof(unsigned int) * kTicketCount) ==
+                hipSuccess &&
             hipDeviceSynchronize() == hipSuccess;
   if (!ok) {
     if (h->d_partials) (void)hipFree(h->d_partials);
@@ -93,6 +96,7 @@ hcblasStatus_t hcblasDestroy(hcblasHandle_t *handle) {
   if (h->gemv_partials.ptr) (void)hipFree(h->gemv_partials.ptr);
   if (h->gemv_tickets.ptr) (void)hipFree(h->gemv_tickets.ptr);
   if (h->gemm_ws.ptr) (void)hipFree(h->gemm_ws.ptr);
+  if (h->split_ws.ptr) (void)hipFree(h->split_ws.ptr);
   delete h;
   *handle = nullptr;
   return HCBLAS_STATUS_SUCCESS;
@@ -160,14 +164,41 @@ hcblasStatus_t copy_strided(hcblasHandle_t h, int rows, int cols, int elem,
                             int64_t dpitch_elems, bool to_device) {
   if (rows <= 0 || cols <= 0) return HCBLAS_STATUS_SUCCESS;
   const void *dev = to_device ? dst : src;
-  int64_t dev_pitch = to_device ? dpitch_elems : spitch_elems;
-  size_t span = (size_t)((int64_t)(cols - 1) * dev_pitch + rows) * elem;
+  const int64_t dev_pitch = to_device ? dpitch_elems : spitch_elems;
+  const int64_t host_pitch = to_device ? spitch_elems : dpitch_elems;
+  const size_t span = (size_t)((int64_t)(cols - 1) * dev_pitch + rows) * elem;
   if (!device_range_ok(dev, span)) return HCBLAS_STATUS_MAPPING_ERROR;
-  hipError_t e = hipMemcpy2DAsync(
-      dst, (size_t)dpitch_elems * elem, src, (size_t)spitch_elems * elem,
-      (size_t)rows * elem, (size_t)cols,
-      to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost, h->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  const hipMemcpyKind kind =
+      to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
+  hipError_t e = hipSuccess;
+  if (dev_pitch == rows && host_pitch == rows) {
+    // Dense on both sides: one copy.
+    e = hipMemcpyAsync(dst, src, (size_t)rows * cols * elem, kind, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  } else {
+    // Strided: stage the device span on the host, scatter / gather there.
+    std::vector<char> stage(span);
+    const size_t rb = (size_t)rows * elem;
+    if (to_device) {
+      e = hipMemcpy(stage.data(), dst, span, hipMemcpyDeviceToHost);
+      if (e == hipSuccess)
+        for (int c = 0; c < cols; ++c)
+          std::memcpy(stage.data() + (size_t)c * dev_pitch * elem,
+                      static_cast<const char *>(src) +
+                          (size_t)c * host_pitch * elem,
+                      rb);
+      if (e == hipSuccess)
+        e = hipMemcpy(dst, stage.data(), span, hipMemcpyHostToDevice);
+    } else {
+      e = hipStreamSynchronize(h->stream);
+      if (e == hipSuccess)
+        e = hipMemcpy(stage.data(), src, span, hipMemcpyDeviceToHost);
+      if (e == hipSuccess)
+        for (int c = 0; c < cols; ++c)
+          std::memcpy(static_cast<char *>(dst) + (size_t)c * host_pitch * elem,
+                      stage.data() + (size_t)c * dev_pitch * elem, rb);
+    }
+  }
   return e == hipSuccess ? HCBLAS_STATUS_SUCCESS
                          : HCBLAS_STATUS_MAPPING_ERROR;
 }

@@ -8,38 +8,55 @@
 //   sasum_HC   lib/src/blas/sasum/sasum_array.cpp:30-143
 //
 // Design: every routine is HBM-bound. Unit-stride, 16-byte-aligned operands
-// take a dwordx4 path (four 16-byte loads in flight per thread, grid-stride
-// over a grid sized to the CU count); everything else takes a scalar path that
-// honours BLAS increments. dot/asum are one launch: per-workgroup partials in
-// FP64, then the last workgroup to take a ticket sums the partials in a fixed
-// order, so the result is deterministic and no host round trip is needed for
-// the reduction itself.
+// take the vector path: one 16-byte vector per thread over a full grid
+// (measured on MI355X to stream faster than grid-stride loops), with
+// non-temporal loads / stores once the operands exceed the caches. Everything
+// else takes a scalar path that honours BLAS increments. dot / asum are one
+// launch: each workgroup reduces to an FP64 partial; the last workgroup of
+// every group of 256 (ticket counter) sums that group in index order, and the
+// last group finisher sums the group partials in index order, so the result
+// is deterministic and no host round trip is needed for the reduction.
 #include "common.hpp"
 
 namespace hcb {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kUnroll = 4;
+// Operands larger than this (bytes) bypass the caches (non-temporal).
+constexpr int64_t kStreamBytes = 64ll << 20;
+// Vector-path reductions: loads in flight per thread and maximum grid.
+constexpr int kReduceUnroll = 4;
+constexpr int kReduceGrid = 2048;
+
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef double d2v __attribute__((ext_vector_type(2)));
 
 template <typename T>
-struct V16;  // 16-byte vector of T
+struct V16;  // native 16-byte vector of T
 template <>
 struct V16<float> {
-  using type = float4;
+  using type = f4v;
   static constexpr int n = 4;
 };
 template <>
 struct V16<double> {
-  using type = double2;
+  using type = d2v;
   static constexpr int n = 2;
 };
 
-__device__ __forceinline__ float &lane(float4 &v, int i) {
-  return reinterpret_cast<float *>(&v)[i];
+template <bool NT, typename V>
+__device__ __forceinline__ V vload(const V *p) {
+  if constexpr (NT)
+    return __builtin_nontemporal_load(p);
+  else
+    return *p;
 }
-__device__ __forceinline__ double &lane(double2 &v, int i) {
-  return reinterpret_cast<double *>(&v)[i];
+template <bool NT, typename V>
+__device__ __forceinline__ void vstore(V *p, V v) {
+  if constexpr (NT)
+    __builtin_nontemporal_store(v, p);
+  else
+    *p = v;
 }
 
 // BLAS element index for an increment (negative increments walk backwards).
@@ -79,39 +96,24 @@ struct AxpyOp<float> {
   static constexpr bool reads_y = true;
 };
 
-// Vector path: x, y 16-byte aligned, unit stride, nvec whole vectors,
-// grid-stride with kUnroll independent vectors per thread per trip.
-template <typename T, typename Op>
+// Vector path: x, y 16-byte aligned, unit stride; one vector per thread
+// (grid-stride only beyond kMaxReduceBlocks workgroups).
+template <typename T, typename Op, bool NT>
 __global__ __launch_bounds__(kThreads) void elementwise_vec_kernel(
     int64_t nvec, const typename V16<T>::type *__restrict__ x,
     typename V16<T>::type *__restrict__ y, Op op) {
   using V = typename V16<T>::type;
   constexpr int VN = V16<T>::n;
   const int64_t stride = (int64_t)gridDim.x * kThreads;
-  int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-  for (; i + (kUnroll - 1) * stride < nvec; i += kUnroll * stride) {
-    V xv[kUnroll], yv[kUnroll];
-#pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      xv[u] = x[i + u * stride];
-      if (Op::reads_y) yv[u] = y[i + u * stride];
-    }
-#pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      V r;
-#pragma unroll
-      for (int e = 0; e < VN; ++e) lane(r, e) = op(lane(xv[u], e), lane(yv[u], e));
-      y[i + u * stride] = r;
-    }
-  }
-  for (; i < nvec; i += stride) {
-    V xv = x[i];
-    V yv;
-    if (Op::reads_y) yv = y[i];
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < nvec;
+       i += stride) {
+    const V xv = vload<NT>(x + i);
+    V yv{};
+    if (Op::reads_y) yv = vload<NT>(y + i);
     V r;
 #pragma unroll
-    for (int e = 0; e < VN; ++e) lane(r, e) = op(lane(xv, e), lane(yv, e));
-    y[i] = r;
+    for (int e = 0; e < VN; ++e) r[e] = op(xv[e], yv[e]);
+    vstore<NT>(y + i, r);
   }
 }
 
@@ -133,40 +135,19 @@ __global__ __launch_bounds__(kThreads) void elementwise_strided_kernel(
   }
 }
 
-// Complex scal. Products are rounded separately (no contraction), so the
-// result is bit-identical to the CPU restatement compiled with
-// -ffp-contract=off: re = ar*xr - ai*xi, im = ar*xi + ai*xr.
-template <typename T>
-struct CScalOp {
-  T ar, ai;
-};
-
-template <typename T>
-__device__ __forceinline__ T mul_rn(T a, T b);
-template <>
-__device__ __forceinline__ float mul_rn(float a, float b) {
-  return __fmul_rn(a, b);
-}
-template <>
-__device__ __forceinline__ double mul_rn(double a, double b) {
-  return __dmul_rn(a, b);
-}
-template <typename T>
-__device__ __forceinline__ T add_rn(T a, T b);
-template <>
-__device__ __forceinline__ float add_rn(float a, float b) {
-  return __fadd_rn(a, b);
-}
-template <>
-__device__ __forceinline__ double add_rn(double a, double b) {
-  return __dadd_rn(a, b);
-}
+// Complex scal. Products are rounded separately (contraction is switched
+// off in the kernel body), so the result is bit-identical to the CPU
+// restatement compiled with -ffp-contract=off:
+// re = ar*xr - ai*xi, im = ar*xi + ai*xr. The arithmetic is written in the
+// kernel body itself: a helper defined outside the pragma's scope would be
+// contracted after inlining.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void cscal_kernel(int64_t n,
                                                          int64_t batch, T *x,
                                                          int64_t incx,
                                                          int64_t bsx, T ar,
                                                          T ai, bool real_alpha) {
+#pragma clang fp contract(off)
   const int64_t total = n * batch;
   const int64_t stride = (int64_t)gridDim.x * kThreads;
   for (int64_t f = (int64_t)blockIdx.x * kThreads + threadIdx.x; f < total;
@@ -175,11 +156,12 @@ __global__ __launch_bounds__(kThreads) void cscal_kernel(int64_t n,
     int64_t ix = 2 * (b * bsx + i * incx);
     T xr = x[ix], xi = x[ix + 1];
     if (real_alpha) {
-      x[ix] = mul_rn(ar, xr);
-      x[ix + 1] = mul_rn(ar, xi);
+      x[ix] = ar * xr;
+      x[ix + 1] = ar * xi;
     } else {
-      x[ix] = add_rn(mul_rn(ar, xr), -mul_rn(ai, xi));
-      x[ix + 1] = add_rn(mul_rn(ar, xi), mul_rn(ai, xr));
+      const T p1 = ar * xr, p2 = ai * xi, p3 = ar * xi, p4 = ai * xr;
+      x[ix] = p1 - p2;
+      x[ix + 1] = p3 + p4;
     }
   }
 }
@@ -213,16 +195,12 @@ __device__ __forceinline__ double wave_sum(double v) {
   for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
 }
-__device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
 
-// Block sum of one double per thread, in a fixed order.
+// Block sum of one double per thread, in a fixed order (valid on thread 0).
 __device__ __forceinline__ double block_sum(double v, double *lds) {
   v = wave_sum(v);
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
   if (l == 0) lds[w] = v;
   __syncthreads();
   double r = 0.0;
@@ -230,46 +208,74 @@ __device__ __forceinline__ double block_sum(double v, double *lds) {
 #pragma unroll
     for (int i = 0; i < kThreads / 64; ++i) r += lds[i];
   }
-  return r;  // valid on thread 0 only
+  return r;
 }
 
-// Second stage shared by both reduction kernels: publish this workgroup's
-// partial; the last workgroup to arrive sums every partial in index order and
-// writes the final value as T. The ticket returns to zero for the next call.
+// Publishes this workgroup's partial and runs the deterministic two-level
+// finish: the last workgroup of each group of kReduceGroup sums the group's
+// partials in index order into a group partial; the last group finisher sums
+// the group partials in index order and writes the result as T. Every ticket
+// it takes is returned to zero.
 template <typename T>
-__device__ void finish_reduction(double partial, double *partials,
-                                 unsigned int *ticket, T *result,
-                                 unsigned int nblocks, double *lds) {
+__device__ void tree_finish(double partial, double *partials,
+                            unsigned int *tickets, T *result, double *lds) {
   __shared__ bool last;
-  const unsigned int bid = blockIdx.x;
+  const unsigned int nb = gridDim.x, bid = blockIdx.x;
+  const unsigned int ngroups = (nb + kReduceGroup - 1) / kReduceGroup;
+  const unsigned int g = bid / kReduceGroup;
+  const unsigned int g0 = g * kReduceGroup;
+  const unsigned int gsize = nb - g0 < (unsigned)kReduceGroup
+                                 ? nb - g0
+                                 : (unsigned)kReduceGroup;
+  double *gpart = partials + kMaxReduceBlocks;
   if (threadIdx.x == 0) {
     __hip_atomic_store(&partials[bid], partial, __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
-    unsigned int t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL,
+    unsigned int t = __hip_atomic_fetch_add(&tickets[1 + g], 1u,
+                                            __ATOMIC_ACQ_REL,
                                             __HIP_MEMORY_SCOPE_AGENT);
-    last = (t == nblocks - 1);
+    last = (t == gsize - 1);
   }
   __syncthreads();
   if (!last) return;
-  double s = 0.0;
-  for (unsigned int i = threadIdx.x; i < nblocks; i += kThreads)
-    s += __hip_atomic_load(&partials[i], __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
+  // Group finisher (kReduceGroup == kThreads: one partial per thread).
+  double s = threadIdx.x < gsize
+                 ? __hip_atomic_load(&partials[g0 + threadIdx.x],
+                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                 : 0.0;
+  double gs = block_sum(s, lds);
+  if (threadIdx.x == 0) {
+    __hip_atomic_store(&tickets[1 + g], 0u, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&gpart[g], gs, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    unsigned int t = __hip_atomic_fetch_add(&tickets[0], 1u, __ATOMIC_ACQ_REL,
+                                            __HIP_MEMORY_SCOPE_AGENT);
+    last = (t == ngroups - 1);
+  }
   __syncthreads();
-  double tot = block_sum(s, lds);
+  if (!last) return;
+  double f = 0.0;
+  for (unsigned int i = threadIdx.x; i < ngroups; i += kThreads)
+    f += __hip_atomic_load(&gpart[i], __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+  double tot = block_sum(f, lds);
   if (threadIdx.x == 0) {
     *result = (T)tot;
-    __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&tickets[0], 0u, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
-// Vector-path reduction over nvec whole 16-byte vectors plus a scalar tail
-// (tail handled by block 0). Accumulates in T per lane of the vector with
-// kUnroll independent chains, then FP64 across lanes and workgroups.
-template <typename T, typename Red>
+// Vector-path reduction: grid-stride over 16-byte vectors of x (and y) with
+// U vectors in flight per thread; the scalar tail (< one vector) is added by
+// block 0. The grid is kept to a few thousand workgroups: every workgroup
+// takes one agent-scope ticket, whose release is an L2 write-back on this
+// multi-XCD part, so a full grid of ~10^5 workgroups would serialise on them.
+template <typename T, typename Red, bool NT, int U>
 __global__ __launch_bounds__(kThreads) void reduce_vec_kernel(
     int64_t n, const T *__restrict__ x, const T *__restrict__ y,
-    double *partials, unsigned int *ticket, T *result, Red red) {
+    double *partials, unsigned int *tickets, T *result, Red red) {
   using V = typename V16<T>::type;
   constexpr int VN = V16<T>::n;
   __shared__ double lds[kThreads / 64];
@@ -277,42 +283,42 @@ __global__ __launch_bounds__(kThreads) void reduce_vec_kernel(
   const V *yv = reinterpret_cast<const V *>(y);
   const int64_t nvec = n / VN;
   const int64_t stride = (int64_t)gridDim.x * kThreads;
-  T acc[kUnroll][VN];
+  T acc[U][VN];
 #pragma unroll
-  for (int u = 0; u < kUnroll; ++u)
+  for (int u = 0; u < U; ++u)
 #pragma unroll
     for (int e = 0; e < VN; ++e) acc[u][e] = T(0);
   int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-  for (; i + (kUnroll - 1) * stride < nvec; i += kUnroll * stride) {
-    V a[kUnroll], b[kUnroll];
+  for (; i + (U - 1) * stride < nvec; i += U * stride) {
+    V a[U], b[U];
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      a[u] = xv[i + u * stride];
-      if (Red::two) b[u] = yv[i + u * stride];
+    for (int u = 0; u < U; ++u) {
+      a[u] = vload<NT>(xv + i + u * stride);
+      if (Red::two) b[u] = vload<NT>(yv + i + u * stride);
     }
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u)
+    for (int u = 0; u < U; ++u)
 #pragma unroll
-      for (int e = 0; e < VN; ++e)
-        acc[u][e] += red(lane(a[u], e), lane(b[u], e));
+      for (int e = 0; e < VN; ++e) acc[u][e] += red(a[u][e], b[u][e]);
   }
   for (; i < nvec; i += stride) {
-    V a = xv[i], b;
-    if (Red::two) b = yv[i];
+    const V a = vload<NT>(xv + i);
+    V b{};
+    if (Red::two) b = vload<NT>(yv + i);
 #pragma unroll
-    for (int e = 0; e < VN; ++e) acc[0][e] += red(lane(a, e), lane(b, e));
-  }
-  if (blockIdx.x == 0) {
-    int64_t t = nvec * VN + threadIdx.x;
-    if (t < n) acc[1][0] += red(x[t], Red::two ? y[t] : T(0));
+    for (int e = 0; e < VN; ++e) acc[0][e] += red(a[e], b[e]);
   }
   double s = 0.0;
 #pragma unroll
-  for (int u = 0; u < kUnroll; ++u)
+  for (int u = 0; u < U; ++u)
 #pragma unroll
     for (int e = 0; e < VN; ++e) s += (double)acc[u][e];
+  if (blockIdx.x == 0) {
+    const int64_t t = nvec * VN + threadIdx.x;
+    if (t < n) s += (double)red(x[t], Red::two ? y[t] : T(0));
+  }
   double part = block_sum(s, lds);
-  finish_reduction<T>(part, partials, ticket, result, gridDim.x, lds);
+  tree_finish<T>(part, partials, tickets, result, lds);
 }
 
 // Strided / batched reduction: element (b, i) at b*bs + blas_index(i, n, inc).
@@ -320,7 +326,7 @@ template <typename T, typename Red>
 __global__ __launch_bounds__(kThreads) void reduce_strided_kernel(
     int64_t n, int64_t batch, const T *__restrict__ x, int64_t incx,
     int64_t bsx, const T *__restrict__ y, int64_t incy, int64_t bsy,
-    double *partials, unsigned int *ticket, T *result, Red red) {
+    double *partials, unsigned int *tickets, T *result, Red red) {
   __shared__ double lds[kThreads / 64];
   const int64_t total = n * batch;
   const int64_t stride = (int64_t)gridDim.x * kThreads;
@@ -333,19 +339,16 @@ __global__ __launch_bounds__(kThreads) void reduce_strided_kernel(
     acc += red(xv, yv);
   }
   double part = block_sum((double)acc, lds);
-  finish_reduction<T>(part, partials, ticket, result, gridDim.x, lds);
+  tree_finish<T>(part, partials, tickets, result, lds);
 }
 
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
-int stream_grid(hcblasHandle_t h, int64_t work_items) {
-  // Enough workgroups to cover every SIMD several times over, never more
-  // than the work needs.
-  int64_t want = cdiv(work_items, (int64_t)kThreads * kUnroll);
-  int64_t cap = (int64_t)h->num_cus * 8;
-  if (want < 1) want = 1;
-  return (int)(want < cap ? want : cap);
+int full_grid(int64_t work_items) {
+  int64_t g = cdiv(work_items, kThreads);
+  if (g < 1) g = 1;
+  return (int)(g < kMaxReduceBlocks ? g : kMaxReduceBlocks);
 }
 
 template <typename T, typename Op>
@@ -353,20 +356,27 @@ hcblasStatus_t run_elementwise(hcblasHandle_t h, int64_t n, int64_t batch,
                                const T *x, int64_t incx, T *y, int64_t incy,
                                Op op) {
   constexpr int VN = V16<T>::n;
+  using V = typename V16<T>::type;
   if (n <= 0 || batch <= 0) return HCBLAS_STATUS_SUCCESS;
   // A unit-stride batch with stride n is one contiguous vector of n*batch.
   const bool contiguous = incx == 1 && incy == 1;
   const int64_t flat = n * batch;
   if (contiguous && aligned16(x) && aligned16(y)) {
-    int64_t nvec = flat / VN;
+    const int64_t nvec = flat / VN;
     if (nvec > 0) {
-      using V = typename V16<T>::type;
-      elementwise_vec_kernel<T, Op>
-          <<<stream_grid(h, nvec), kThreads, 0, h->stream>>>(
-              nvec, reinterpret_cast<const V *>(x), reinterpret_cast<V *>(y),
-              op);
+      const bool nt = flat * (int64_t)sizeof(T) >= kStreamBytes;
+      if (nt)
+        elementwise_vec_kernel<T, Op, true>
+            <<<full_grid(nvec), kThreads, 0, h->stream>>>(
+                nvec, reinterpret_cast<const V *>(x), reinterpret_cast<V *>(y),
+                op);
+      else
+        elementwise_vec_kernel<T, Op, false>
+            <<<full_grid(nvec), kThreads, 0, h->stream>>>(
+                nvec, reinterpret_cast<const V *>(x), reinterpret_cast<V *>(y),
+                op);
     }
-    int64_t tail = flat - nvec * VN;
+    const int64_t tail = flat - nvec * VN;
     if (tail > 0)
       elementwise_strided_kernel<T, Op><<<1, kThreads, 0, h->stream>>>(
           tail, 1, x + nvec * VN, 1, 0, y + nvec * VN, 1, 0, op);
@@ -374,13 +384,13 @@ hcblasStatus_t run_elementwise(hcblasHandle_t h, int64_t n, int64_t batch,
   }
   if (contiguous) {
     elementwise_strided_kernel<T, Op>
-        <<<stream_grid(h, flat), kThreads, 0, h->stream>>>(flat, 1, x, 1, 0, y,
-                                                           1, 0, op);
+        <<<full_grid(flat), kThreads, 0, h->stream>>>(flat, 1, x, 1, 0, y, 1,
+                                                      0, op);
     return launch_status();
   }
   elementwise_strided_kernel<T, Op>
-      <<<stream_grid(h, flat), kThreads, 0, h->stream>>>(n, batch, x, incx, n,
-                                                         y, incy, n, op);
+      <<<full_grid(flat), kThreads, 0, h->stream>>>(n, batch, x, incx, n, y,
+                                                    incy, n, op);
   return launch_status();
 }
 
@@ -391,20 +401,32 @@ hcblasStatus_t run_reduce(hcblasHandle_t h, int64_t n, int64_t batch,
   constexpr int VN = V16<T>::n;
   T *dres = reinterpret_cast<T *>(h->d_result);
   const int64_t flat = n * batch;
-  int grid = stream_grid(h, flat / VN);
-  if (grid > kMaxPartials) grid = kMaxPartials;
   const bool contiguous = incx == 1 && (!Red::two || incy == 1);
+  const T *yy = Red::two ? y : x;
   if (contiguous && aligned16(x) && (!Red::two || aligned16(y))) {
-    reduce_vec_kernel<T, Red><<<grid, kThreads, 0, h->stream>>>(
-        flat, x, Red::two ? y : x, h->d_partials, h->d_ticket, dres, red);
+    const int64_t nvec = flat / VN;
+    const int grid = (int)std::max<int64_t>(
+        1, std::min<int64_t>(cdiv(nvec, kThreads * kReduceUnroll),
+                             kReduceGrid));
+    const int64_t bytes = flat * (int64_t)sizeof(T) * (Red::two ? 2 : 1);
+    if (bytes >= kStreamBytes)
+      reduce_vec_kernel<T, Red, true, kReduceUnroll>
+          <<<grid, kThreads, 0, h->stream>>>(flat, x, yy, h->d_partials,
+                                             h->d_ticket, dres, red);
+    else
+      reduce_vec_kernel<T, Red, false, kReduceUnroll>
+          <<<grid, kThreads, 0, h->stream>>>(flat, x, yy, h->d_partials,
+                                             h->d_ticket, dres, red);
   } else if (contiguous) {
-    reduce_strided_kernel<T, Red><<<grid, kThreads, 0, h->stream>>>(
-        flat, 1, x, 1, 0, Red::two ? y : x, 1, 0, h->d_partials, h->d_ticket,
-        dres, red);
+    reduce_strided_kernel<T, Red>
+        <<<(int)std::min<int64_t>(full_grid(flat), kReduceGrid), kThreads, 0,
+           h->stream>>>(flat, 1, x, 1, 0, yy, 1, 0, h->d_partials,
+                        h->d_ticket, dres, red);
   } else {
-    reduce_strided_kernel<T, Red><<<grid, kThreads, 0, h->stream>>>(
-        n, batch, x, incx, n, Red::two ? y : x, incy, n, h->d_partials,
-        h->d_ticket, dres, red);
+    reduce_strided_kernel<T, Red>
+        <<<(int)std::min<int64_t>(full_grid(flat), kReduceGrid), kThreads, 0,
+           h->stream>>>(n, batch, x, incx, n, yy, incy, n, h->d_partials,
+                        h->d_ticket, dres, red);
   }
   hcblasStatus_t st = launch_status();
   if (st != HCBLAS_STATUS_SUCCESS) return st;
@@ -452,7 +474,7 @@ hcblasStatus_t l1_cscal(hcblasHandle_t h, int n, T ar, T ai, bool real_alpha,
   if (n <= 0 || batch == 0 || incx <= 0) return HCBLAS_STATUS_SUCCESS;
   if (x == nullptr) return HCBLAS_STATUS_INVALID_VALUE;
   int64_t total = (int64_t)n * batch;
-  cscal_kernel<T><<<stream_grid(h, total), kThreads, 0, h->stream>>>(
+  cscal_kernel<T><<<full_grid(total), kThreads, 0, h->stream>>>(
       n, batch, x, incx, n, ar, ai, real_alpha);
   return launch_status();
 }

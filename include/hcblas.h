@@ -131,7 +131,7 @@ hcblasStatus_t hcblasSetMathMode(hcblasHandle_t handle, int mode);
 hcblasStatus_t hcblasGetMathMode(hcblasHandle_t handle, int *mode);
 
 /* Number of INT8 slices per operand for the Ozaki path, 3..12
- * (no reference counterpart). Default 7. */
+ * (no reference counterpart). Default 8. */
 hcblasStatus_t hcblasSetOzakiSlices(hcblasHandle_t handle, int slices);
 
 /* ------------------------------------------------------------------------ */

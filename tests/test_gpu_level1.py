@@ -182,13 +182,17 @@ def test_complex_scal_bit_exact(hc, handle, orc, p):
     real = getattr(api, "hcblasCsscal" if p == "c" else "hcblasZdscal")
     assert real(handle, n, 1.5, dx, 1) == 0
     assert bitwise_equal(to_host(dx), orc.scal(1.5, x))
-    # batched, stride n, incx 2 inside each batch
-    dx = to_dev(np.concatenate([x, x]))
+    # batched: batch stride n (hcblas.cpp:376), two batches
+    xx = np.concatenate([x, x[::-1]])
+    dx = to_dev(xx)
     bat = getattr(api, f"hcblas{p.upper()}scalBatched")
-    assert bat(handle, n // 2, a, dx, 2, 2) == 0
-    got = to_host(dx)
-    exp0 = orc.scal(a, x, 2, n // 2)
-    assert bitwise_equal(got[:n], exp0)
+    assert bat(handle, n, a, dx, 1, 2) == 0
+    assert bitwise_equal(to_host(dx), orc.scal(a, xx))
+    # strided inside one vector
+    dx = to_dev(x)
+    fn = getattr(api, f"hcblas{p.upper()}scal")
+    assert fn(handle, n // 2, a, dx, 2) == 0
+    assert bitwise_equal(to_host(dx), orc.scal(a, x, 2, n // 2))
 
 
 def test_full_size_properties(hc, handle, torch_cuda):
@@ -209,9 +213,10 @@ def test_full_size_properties(hc, handle, torch_cuda):
     # because every partial is an integer below 2^24)
     st, r = api.hcblasSdot(handle, n, x, 1, y, 1, None)
     exact = float((x.double() * y.double()).sum())
-    assert st == 0 and r == exact
+    # the result is an fp32 scalar: exact value rounded once
+    assert st == 0 and r == float(np.float32(exact))
     st, r = api.hcblasSasum(handle, n, x, 1, None)
-    assert st == 0 and r == float(x.double().abs().sum())
+    assert st == 0 and r == float(np.float32(x.double().abs().sum().item()))
     assert api.hcblasSaxpy(handle, n, 2.0, x, 1, z, 1) == 0
     torch.cuda.synchronize()
     assert torch.equal(z, 3 * x)

@@ -1,0 +1,204 @@
+"""Split-precision GEMM on the GPU (no reference counterpart):
+BF16 three-way split SGEMM / CGEMM and Ozaki INT8 DGEMM / ZGEMM.
+
+Checks:
+  - bit-exact on the reference's integer recipes (test/unit/hcblas_sgemm.cpp
+    189x9x19, all order x trans cases): small integers split / slice exactly;
+  - every order x trans case within the plain kernel's error bound of the
+    double-double oracle;
+  - measured error against the double-double oracle no worse than the plain
+    FP32 / FP64 MFMA kernel on the same inputs, for uniform[-1,1] and for the
+    wide-exponent distribution sign*2^u, u ~ U[-20, 20];
+  - the handle's math mode routes hcblasSgemm / hcblasDgemm / C / Z GEMM to
+    the split kernels with results identical to the explicit entries.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from gpu_util import bitwise_equal, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+ROW, COL = 0, 1
+N_, T_, C_ = 0, 1, 2
+
+
+def _split(hc, handle, p, o, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C,
+           ldc):
+    api = hc.api
+    dA, dB, dC = to_dev(A), to_dev(B), to_dev(C)
+    fn = api.hcblas_sgemm_split_bf16 if p == "s" else api.hcblas_dgemm_ozaki_int8
+    st = fn(handle, o, ta, tb, M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc,
+            0, 0, 0)
+    assert st == 0
+    return to_host(dC)
+
+
+def _plain(hc, handle, p, o, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C,
+           ldc):
+    api = hc.api
+    dA, dB, dC = to_dev(A), to_dev(B), to_dev(C)
+    st = getattr(api, f"hcblas_{p}gemm")(handle, o, ta, tb, M, N, K, alpha, dA,
+                                         lda, dB, ldb, beta, dC, ldc, 0, 0, 0)
+    assert st == 0
+    return to_host(dC)
+
+
+@pytest.mark.parametrize("p", ["s", "d"])
+def test_reference_recipe_bit_exact(hc, handle, golden, p):
+    g = golden("gemm_ref_int.npz")
+    M, N, K = (int(v) for v in g["dims"])
+    for i in range(8):
+        o, ta, tb, lda, ldb, ldc = (int(v) for v in g[f"{p}_case{i}"])
+        got = _split(hc, handle, p, o, ta, tb, M, N, K, 1.0, g[f"{p}_A"], lda,
+                     g[f"{p}_B"], ldb, 1.0, g[f"{p}_C"], ldc)
+        assert bitwise_equal(got, g[f"{p}_expect{i}"]), f"case {i}"
+
+
+def _gen(orc, p, n, seed, wide):
+    if p == "s":
+        return orc.wide_f32(n, seed) if wide else orc.uniform_f32(n, seed)
+    return orc.wide_f64(n, seed) if wide else orc.uniform_f64(n, seed)
+
+
+def _rel_errors(orc, o, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
+                got, rows, cols):
+    ref = orc.gemm_sample_exact(o, ta, tb, M, N, K, alpha, A, lda, B, ldb,
+                                beta, C, ldc, rows, cols)
+    idx = rows + cols * ldc if o == COL else rows * ldc + cols
+    return np.abs(got[idx] - ref) / np.maximum(np.abs(ref), 1e-300)
+
+
+@pytest.mark.parametrize("p", ["s", "d"])
+@pytest.mark.parametrize("M,N,K", [(33, 17, 45), (128, 256, 512), (300, 129, 1000)])
+def test_all_cases_within_plain_bound(hc, handle, orc, p, M, N, K):
+    eps = np.finfo(np.float32 if p == "s" else np.float64).eps
+    for o in (COL, ROW):
+        for ta in (N_, T_):
+            for tb in (N_, T_):
+                ar, ac = (M, K) if ta == N_ else (K, M)
+                br, bc = (K, N) if tb == N_ else (N, K)
+                lda = (ar if o == COL else ac) + 1
+                ldb = (br if o == COL else bc) + 2
+                ldc = (M if o == COL else N)
+                A = _gen(orc, p, lda * (ac if o == COL else ar), 1, False)
+                B = _gen(orc, p, ldb * (bc if o == COL else br), 2, False)
+                C = _gen(orc, p, ldc * (N if o == COL else M), 3, False)
+                got = _split(hc, handle, p, o, ta, tb, M, N, K, 0.5, A, lda, B,
+                             ldb, -1.5, C, ldc)
+                rng = np.random.default_rng(0)
+                rows = rng.integers(0, M, 256)
+                cols = rng.integers(0, N, 256)
+                ref = orc.gemm_sample_exact(o, ta, tb, M, N, K, 0.5, A, lda, B,
+                                            ldb, -1.5, C, ldc, rows, cols)
+                idx = rows + cols * ldc if o == COL else rows * ldc + cols
+                tol = 2 * (K + 2) * eps * (0.5 * K + 1.5 + 1)
+                assert np.max(np.abs(got[idx] - ref)) <= tol, (o, ta, tb)
+
+
+@pytest.mark.parametrize("p", ["s", "d"])
+@pytest.mark.parametrize("wide", [False, True])
+@pytest.mark.parametrize("ta", [N_, T_])
+def test_error_no_worse_than_plain(hc, handle, orc, p, wide, ta):
+    """Measured max / mean relative error vs the double-double oracle on
+    sampled entries: split <= plain (K = 4096)."""
+    M = N = 384
+    K = 4096
+    lda = M if ta == N_ else K
+    A = _gen(orc, p, M * K, 11, wide)
+    B = _gen(orc, p, K * N, 12, wide)
+    C = np.zeros(M * N, A.dtype)
+    rng = np.random.default_rng(1)
+    rows = rng.integers(0, M, 1024)
+    cols = rng.integers(0, N, 1024)
+    plain = _plain(hc, handle, p, COL, ta, N_, M, N, K, 1.0, A, lda, B, K, 0.0,
+                   C, M)
+    split = _split(hc, handle, p, COL, ta, N_, M, N, K, 1.0, A, lda, B, K, 0.0,
+                   C, M)
+    ep = _rel_errors(orc, COL, ta, N_, M, N, K, 1.0, A, lda, B, K, 0.0, C, M,
+                     plain, rows, cols)
+    es = _rel_errors(orc, COL, ta, N_, M, N, K, 1.0, A, lda, B, K, 0.0, C, M,
+                     split, rows, cols)
+    print(f"{p} wide={wide} ta={ta}: plain max {ep.max():.3e} mean {ep.mean():.3e}"
+          f" | split max {es.max():.3e} mean {es.mean():.3e}")
+    assert es.mean() <= ep.mean()
+    assert np.median(es) <= np.median(ep)
+
+
+def test_math_mode_routes_plain_entries(hc, handle, orc):
+    lib = hc.load()
+    api = hc.api
+    M, N, K = 200, 150, 300
+    for p, mode in (("s", 1), ("d", 2)):
+        A = _gen(orc, p, M * K, 5, False)
+        B = _gen(orc, p, K * N, 6, False)
+        C = _gen(orc, p, M * N, 7, False)
+        exp = _split(hc, handle, p, COL, N_, T_, M, N, K, 1.25, A, M,
+                     B[:N * K], N, 0.5, C, M)
+        assert lib.hcblasSetMathMode(handle.value, mode) == 0
+        dC = to_dev(C)
+        st = getattr(api, f"hcblas{p.upper()}gemm")(handle, N_, T_, M, N, K,
+                                                    1.25, to_dev(A), M,
+                                                    to_dev(B[:N * K]), N, 0.5,
+                                                    dC, M)
+        assert st == 0
+        assert bitwise_equal(to_host(dC), exp)
+        assert lib.hcblasSetMathMode(handle.value, 0) == 0
+
+
+@pytest.mark.parametrize("p", ["c", "z"])
+def test_complex_split_modes(hc, handle, orc, p):
+    lib = hc.load()
+    api = hc.api
+    M, N, K = 96, 80, 200
+    gen = orc.uniform_f32 if p == "c" else orc.uniform_f64
+    dt = np.complex64 if p == "c" else np.complex128
+
+    def cx(n, seed):
+        v = gen(2 * n, seed)
+        return (v[0::2] + 1j * v[1::2]).astype(dt)
+
+    A, B, C = cx(M * K, 1), cx(K * N, 2), cx(M * N, 3)
+    alpha, beta = 0.5 - 1j, 0.25 + 0.5j
+    assert lib.hcblasSetMathMode(handle.value, 1 if p == "c" else 2) == 0
+    for ta in (N_, C_):
+        dC = to_dev(C)
+        lda = M if ta == N_ else K
+        st = getattr(api, f"hcblas{p.upper()}gemm")(handle, ta, N_, M, N, K,
+                                                    alpha, to_dev(A), lda,
+                                                    to_dev(B), K, beta, dC, M)
+        assert st == 0
+        got = to_host(dC)
+        rng = np.random.default_rng(2)
+        rows = rng.integers(0, M, 256)
+        cols = rng.integers(0, N, 256)
+        ref = orc.gemm_sample_exact(COL, ta, N_, M, N, K, alpha, A, lda, B, K,
+                                    beta, C, M, rows, cols)
+        eps = np.finfo(dt).eps
+        tol = 2 * (K + 2) * eps * (abs(alpha) * K * 2 + abs(beta) * 2 + 1)
+        assert np.max(np.abs(got[rows + cols * M] - ref)) <= tol
+    assert lib.hcblasSetMathMode(handle.value, 0) == 0
+
+
+def test_ozaki_slices_control_accuracy(hc, handle, orc):
+    lib = hc.load()
+    M = N = 128
+    K = 2048
+    A = orc.uniform_f64(M * K, 1)
+    B = orc.uniform_f64(K * N, 2)
+    C = np.zeros(M * N)
+    rng = np.random.default_rng(3)
+    rows = rng.integers(0, M, 512)
+    cols = rng.integers(0, N, 512)
+    errs = []
+    for s in (4, 6, 8, 10):
+        assert lib.hcblasSetOzakiSlices(handle.value, s) == 0
+        got = _split(hc, handle, "d", COL, N_, N_, M, N, K, 1.0, A, M, B, K,
+                     0.0, C, M)
+        errs.append(_rel_errors(orc, COL, N_, N_, M, N, K, 1.0, A, M, B, K,
+                                0.0, C, M, got, rows, cols).max())
+    assert lib.hcblasSetOzakiSlices(handle.value, 8) == 0
+    print("ozaki max rel err by slices 4,6,8,10:", errs)
+    assert errs[0] > errs[1] > errs[2]
+    assert errs[3] <= errs[2]

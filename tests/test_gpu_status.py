@@ -81,7 +81,10 @@ def test_argument_errors(hc, handle, torch_cuda):
 
 
 def _hip():
-    lib = ctypes.CDLL("libamdhip64.so")
+    # By SONAME, so this resolves to the HIP runtime already loaded in the
+    # process (torch's), the one libhcblas is bound to; the unversioned name
+    # would load a second, separate runtime from /opt/rocm.
+    lib = ctypes.CDLL("libamdhip64.so.7")
     lib.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
     lib.hipFree.argtypes = [ctypes.c_void_p]
     return lib
