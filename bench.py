@@ -212,6 +212,110 @@ def gemm_section(hc, h, torch, oracle, n, reps=5):
     return out
 
 
+def grouped_problem_table(torch, hc, count=4096, seed=2024):
+    """The grouped-GEMM configuration: `count` independent NN problems with
+    M, N, K each log-uniform on [16, 512] rounded to a multiple of 8, half
+    fp32 SGEMM and half fp16 HGEMM (fp32 accumulate), row- or column-major at
+    random, entries uniform[-1,1] (seeded), alpha = 1, beta = 0.
+    Returns (problems, meta, arenas); `meta` rows are
+    (dtype, order, M, N, K, a_off, b_off, c_off, lda, ldb, ldc) in elements of
+    the dtype's arena."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+
+    def logu(n):
+        v = np.exp(rng.uniform(np.log(16), np.log(512), n))
+        return (np.round(v / 8) * 8).astype(np.int64)
+
+    M, N, K = logu(count), logu(count), logu(count)
+    dt = rng.integers(0, 2, count)
+    order = rng.integers(0, 2, count)
+    sizes = [0, 0]
+    meta = []
+    for i in range(count):
+        d = int(dt[i])
+        al = 64  # elements; keeps every matrix 256-byte aligned
+        offs = []
+        for n_el in (M[i] * K[i], K[i] * N[i], M[i] * N[i]):
+            offs.append(sizes[d])
+            sizes[d] += int((n_el + al - 1) // al * al)
+        row = order[i] == 0
+        lda = K[i] if row else M[i]
+        ldb = N[i] if row else K[i]
+        ldc = N[i] if row else M[i]
+        meta.append((d, int(order[i]), int(M[i]), int(N[i]), int(K[i]),
+                     offs[0], offs[1], offs[2], int(lda), int(ldb), int(ldc)))
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    ar32 = torch.rand(max(1, sizes[0]), generator=g, device="cuda") * 2 - 1
+    ar16 = (torch.rand(max(1, sizes[1]), generator=g, device="cuda") * 2 - 1).half()
+    arenas = (ar32, ar16)
+    problems = []
+    for (d, o, m, n, k, ao, bo, co, lda, ldb, ldc) in meta:
+        base = arenas[d].data_ptr()
+        es = 4 if d == 0 else 2
+        problems.append(hc.GemmProblem(
+            hc.R_32F if d == 0 else hc.R_16F, o, hc.OP_N, hc.OP_N, m, n, k, 0,
+            1.0, 0.0, base + ao * es, lda, base + bo * es, ldb,
+            base + co * es, ldc))
+    return problems, meta, arenas
+
+
+def run_plain_group(hc, h, meta, arenas, c_arenas):
+    """The same problems as per-problem plain launches (hcblas_sgemm /
+    hcblas_hgemm), writing C into `c_arenas`."""
+    api = hc.api
+    for (d, o, m, n, k, ao, bo, co, lda, ldb, ldc) in meta:
+        es = 4 if d == 0 else 2
+        A = arenas[d].data_ptr() + ao * es
+        B = arenas[d].data_ptr() + bo * es
+        C = c_arenas[d].data_ptr() + co * es
+        fn = api.hcblas_sgemm if d == 0 else api.hcblas_hgemm
+        st = fn(h, o, 0, 0, m, n, k, 1.0, A, lda, B, ldb, 0.0, C, ldc, 0, 0, 0)
+        if st != 0:
+            raise RuntimeError(f"plain launch status {st}")
+
+
+def grouped_section(hc, h, torch, count=4096, reps=10):
+    """Grouped GEMM: one autotuned, graph-captured persistent launch versus
+    per-problem plain launches on the same 4096-problem table."""
+    problems, meta, arenas = grouped_problem_table(torch, hc, count)
+    flops32 = sum(2.0 * m * n * k for (d, o, m, n, k, *_) in meta if d == 0)
+    flops16 = sum(2.0 * m * n * k for (d, o, m, n, k, *_) in meta if d == 1)
+    plan = hc.GroupedPlan(h, problems)
+    if plan.autotune(3) != 0:
+        raise RuntimeError("autotune failed")
+    info = plan.info()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    plan.run()
+    torch.cuda.synchronize()
+    s.record()
+    for _ in range(reps):
+        plan.run()
+    e.record()
+    torch.cuda.synchronize()
+    ms_g = s.elapsed_time(e) / reps
+    c_ar = (torch.empty_like(arenas[0]), torch.empty_like(arenas[1]))
+    run_plain_group(hc, h, meta, arenas, c_ar)
+    torch.cuda.synchronize()
+    s.record()
+    for _ in range(3):
+        run_plain_group(hc, h, meta, arenas, c_ar)
+    e.record()
+    torch.cuda.synchronize()
+    ms_p = s.elapsed_time(e) / 3
+    plan.destroy()
+    bound_ms = (flops32 / (PEAK_TFLOPS["fp32"] * 1e12) +
+                flops16 / (PEAK_TFLOPS["bf16"] * 1e12)) * 1e3
+    tot = flops32 + flops16
+    return {"problems": count, "flop": tot, "fp32_flop": flops32,
+            "grouped_ms": round(ms_g, 4), "grouped_tflops": round(tot / ms_g / 1e9, 2),
+            "plain_launches_ms": round(ms_p, 3),
+            "plain_launches_tflops": round(tot / ms_p / 1e9, 2),
+            "speedup": round(ms_p / ms_g, 2),
+            "mfma_bound_ms": round(bound_ms, 4),
+            "roofline_frac": round(bound_ms / ms_g, 4), "plan": info}
+
+
 def cpu_baseline(oracle):
     """The oracle's C restatement (single thread) on a bounded sample of the
     bandwidth set: gemv/ger at 8192^2, level 1 at n = 2^25 / 2^24."""

@@ -8,11 +8,12 @@ not built -- there is no CPU fallback.
 from ._lib import LIB_PATH, HcblasLibraryMissing, load
 from .api import (C_32F, C_64F, COL_MAJOR, DEFAULT_MATH, OP_C, OP_N, OP_T,
                   OZAKI_INT8_MATH, R_16BF, R_16F, R_32F, R_64F, ROW_MAJOR,
-                  SPLIT_BF16_MATH, Handle, HcblasError, Status, api, check,
-                  half_bits)
+                  SPLIT_BF16_MATH, GemmProblem, GroupedPlan, Handle,
+                  HcblasError, Status, api, check, half_bits)
 
 __all__ = [
     "LIB_PATH", "HcblasLibraryMissing", "load", "Handle", "HcblasError",
+    "GemmProblem", "GroupedPlan",
     "Status", "api", "check", "half_bits", "OP_N", "OP_T", "OP_C",
     "ROW_MAJOR", "COL_MAJOR", "DEFAULT_MATH", "SPLIT_BF16_MATH",
     "OZAKI_INT8_MATH", "R_32F", "R_64F", "C_32F", "C_64F", "R_16F", "R_16BF",

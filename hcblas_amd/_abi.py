@@ -40,7 +40,7 @@ def _ctype(ctype: str):
         return ctypes.c_int
     if t == "int64_t":
         return ctypes.c_int64
-    if t in ("hcblasHandle_t", "hipStream_t"):
+    if t in ("hcblasHandle_t", "hipStream_t", "hcblasGroupedPlan_t"):
         return ctypes.c_void_p
     raise ValueError(f"unmapped C type: {ctype!r}")
 

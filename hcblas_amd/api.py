@@ -44,6 +44,56 @@ DEFAULT_MATH, SPLIT_BF16_MATH, OZAKI_INT8_MATH = 0, 1, 2
 R_32F, R_64F, C_32F, C_64F, R_16F, R_16BF = 0, 1, 2, 3, 4, 5
 
 
+class GemmProblem(ctypes.Structure):
+    """hcblasGemmProblem of include/hcblas.h (one entry of a grouped GEMM)."""
+    _fields_ = [("dtype", ctypes.c_int32), ("order", ctypes.c_int32),
+                ("transA", ctypes.c_int32), ("transB", ctypes.c_int32),
+                ("M", ctypes.c_int32), ("N", ctypes.c_int32),
+                ("K", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("alpha", ctypes.c_float), ("beta", ctypes.c_float),
+                ("A", ctypes.c_void_p), ("lda", ctypes.c_int64),
+                ("B", ctypes.c_void_p), ("ldb", ctypes.c_int64),
+                ("C", ctypes.c_void_p), ("ldc", ctypes.c_int64)]
+
+
+class GroupedPlan:
+    """Owns an hcblasGroupedPlan_t built from a list of GemmProblem."""
+
+    def __init__(self, handle, problems):
+        lib = _lib.load()
+        self._table = (GemmProblem * len(problems))(*problems)
+        self._p = ctypes.c_void_p()
+        st = lib.hcblasGroupedPlanCreate(handle.value, self._table,
+                                         len(problems), ctypes.byref(self._p))
+        if st != Status.SUCCESS:
+            raise HcblasError("hcblasGroupedPlanCreate", st)
+        self.handle = handle
+
+    @property
+    def value(self):
+        return self._p.value
+
+    def run(self):
+        return _lib.load().hcblasGroupedGemm(self.handle.value, self._p)
+
+    def autotune(self, reps=3):
+        return _lib.load().hcblasGroupedPlanAutotune(self.handle.value,
+                                                     self._p, reps)
+
+    def info(self):
+        vals = [ctypes.c_int() for _ in range(4)]
+        st = _lib.load().hcblasGroupedPlanInfo(self._p, *[ctypes.byref(v) for v in vals])
+        if st != Status.SUCCESS:
+            raise HcblasError("hcblasGroupedPlanInfo", st)
+        return dict(zip(("tile", "order", "tiles", "workgroups"),
+                        (v.value for v in vals)))
+
+    def destroy(self):
+        if self._p.value:
+            _lib.load().hcblasGroupedPlanDestroy(self._p)
+            self._p = ctypes.c_void_p()
+
+
 class HcblasError(RuntimeError):
     def __init__(self, name, status):
         super().__init__(f"{name} returned {Status(status).name}")

@@ -93,75 +93,80 @@ __device__ __forceinline__ To bitcast(const From &f) {
   return t;
 }
 
-// One MFMA step s of a fragment pair.
-template <Elem E, int S>
-__device__ __forceinline__ void mfma_step(const uint4 &a, const uint4 &b,
-                                          typename ElemT<E>::AccV &c);
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef int intx16 __attribute__((ext_vector_type(16)));
 
+// Accumulator of one MI x MI MFMA block (MI = 16 or 32).
+template <Elem E, int MI>
+struct AccOf;
 template <>
-__device__ __forceinline__ void mfma_step<Elem::F32, 0>(const uint4 &a,
-                                                        const uint4 &b,
-                                                        floatx4 &c) {
-  c = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(a.x),
-                                           __uint_as_float(b.x), c, 0, 0, 0);
-}
+struct AccOf<Elem::F32, 16> { using type = floatx4; };
 template <>
-__device__ __forceinline__ void mfma_step<Elem::F32, 1>(const uint4 &a,
-                                                        const uint4 &b,
-                                                        floatx4 &c) {
-  c = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(a.y),
-                                           __uint_as_float(b.y), c, 0, 0, 0);
-}
+struct AccOf<Elem::F32, 32> { using type = floatx16; };
 template <>
-__device__ __forceinline__ void mfma_step<Elem::F32, 2>(const uint4 &a,
-                                                        const uint4 &b,
-                                                        floatx4 &c) {
-  c = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(a.z),
-                                           __uint_as_float(b.z), c, 0, 0, 0);
-}
+struct AccOf<Elem::F64, 16> { using type = doublex4; };
 template <>
-__device__ __forceinline__ void mfma_step<Elem::F32, 3>(const uint4 &a,
-                                                        const uint4 &b,
-                                                        floatx4 &c) {
-  c = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(a.w),
-                                           __uint_as_float(b.w), c, 0, 0, 0);
-}
+struct AccOf<Elem::F16, 16> { using type = floatx4; };
 template <>
-__device__ __forceinline__ void mfma_step<Elem::F64, 0>(const uint4 &a,
-                                                        const uint4 &b,
-                                                        doublex4 &c) {
-  double av = bitcast<double>(make_uint2(a.x, a.y));
-  double bv = bitcast<double>(make_uint2(b.x, b.y));
-  c = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, c, 0, 0, 0);
-}
+struct AccOf<Elem::F16, 32> { using type = floatx16; };
 template <>
-__device__ __forceinline__ void mfma_step<Elem::F64, 1>(const uint4 &a,
-                                                        const uint4 &b,
-                                                        doublex4 &c) {
-  double av = bitcast<double>(make_uint2(a.z, a.w));
-  double bv = bitcast<double>(make_uint2(b.z, b.w));
-  c = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, c, 0, 0, 0);
-}
+struct AccOf<Elem::BF16, 16> { using type = floatx4; };
 template <>
-__device__ __forceinline__ void mfma_step<Elem::F16, 0>(const uint4 &a,
-                                                        const uint4 &b,
-                                                        floatx4 &c) {
-  c = __builtin_amdgcn_mfma_f32_16x16x32_f16(bitcast<halfx8>(a),
-                                             bitcast<halfx8>(b), c, 0, 0, 0);
-}
+struct AccOf<Elem::BF16, 32> { using type = floatx16; };
 template <>
-__device__ __forceinline__ void mfma_step<Elem::BF16, 0>(const uint4 &a,
-                                                         const uint4 &b,
-                                                         floatx4 &c) {
-  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bitcast<bf16x8>(a),
-                                              bitcast<bf16x8>(b), c, 0, 0, 0);
-}
+struct AccOf<Elem::I8, 16> { using type = intx4; };
 template <>
-__device__ __forceinline__ void mfma_step<Elem::I8, 0>(const uint4 &a,
-                                                       const uint4 &b,
-                                                       intx4 &c) {
-  c = __builtin_amdgcn_mfma_i32_16x16x64_i8(bitcast<intx4>(a),
-                                            bitcast<intx4>(b), c, 0, 0, 0);
+struct AccOf<Elem::I8, 32> { using type = intx16; };
+
+// MFMA step S of a fragment pair for MI x MI blocks:
+//   FP32  v_mfma_f32_16x16x4f32 / v_mfma_f32_32x32x2f32 (one float per step)
+//   FP64  v_mfma_f64_16x16x4f64                         (one double per step)
+//   FP16  v_mfma_f32_16x16x32_f16 / v_mfma_f32_32x32x16_f16
+//   BF16  v_mfma_f32_16x16x32_bf16 / v_mfma_f32_32x32x16_bf16
+//   INT8  v_mfma_i32_16x16x64_i8 / v_mfma_i32_32x32x32_i8
+template <Elem E, int MI, int S>
+__device__ __forceinline__ void mfma_step(const uint4 &a, const uint4 &b,
+                                          typename AccOf<E, MI>::type &c) {
+  if constexpr (E == Elem::F32) {
+    const uint32_t ua = S == 0 ? a.x : S == 1 ? a.y : S == 2 ? a.z : a.w;
+    const uint32_t ub = S == 0 ? b.x : S == 1 ? b.y : S == 2 ? b.z : b.w;
+    if constexpr (MI == 16)
+      c = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(ua),
+                                               __uint_as_float(ub), c, 0, 0, 0);
+    else
+      c = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(ua),
+                                               __uint_as_float(ub), c, 0, 0, 0);
+  } else if constexpr (E == Elem::F64) {
+    static_assert(MI == 16, "FP64 MFMA is 16x16 only");
+    const double av =
+        bitcast<double>(S == 0 ? make_uint2(a.x, a.y) : make_uint2(a.z, a.w));
+    const double bv =
+        bitcast<double>(S == 0 ? make_uint2(b.x, b.y) : make_uint2(b.z, b.w));
+    c = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, c, 0, 0, 0);
+  } else if constexpr (E == Elem::F16) {
+    if constexpr (MI == 16)
+      c = __builtin_amdgcn_mfma_f32_16x16x32_f16(bitcast<halfx8>(a),
+                                                 bitcast<halfx8>(b), c, 0, 0, 0);
+    else
+      c = __builtin_amdgcn_mfma_f32_32x32x16_f16(bitcast<halfx8>(a),
+                                                 bitcast<halfx8>(b), c, 0, 0, 0);
+  } else if constexpr (E == Elem::BF16) {
+    if constexpr (MI == 16)
+      c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bitcast<bf16x8>(a),
+                                                  bitcast<bf16x8>(b), c, 0, 0,
+                                                  0);
+    else
+      c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bitcast<bf16x8>(a),
+                                                  bitcast<bf16x8>(b), c, 0, 0,
+                                                  0);
+  } else {
+    if constexpr (MI == 16)
+      c = __builtin_amdgcn_mfma_i32_16x16x64_i8(bitcast<intx4>(a),
+                                                bitcast<intx4>(b), c, 0, 0, 0);
+    else
+      c = __builtin_amdgcn_mfma_i32_32x32x32_i8(bitcast<intx4>(a),
+                                                bitcast<intx4>(b), c, 0, 0, 0);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -273,10 +278,15 @@ __device__ __forceinline__ void tile_coords(int pid, int tm, int tn,
   tile_n = in_group / gsize;
 }
 
-// Row (within a 16x16 MFMA block) of accumulator value r held by lane group g.
-template <Elem E>
+// Row (within an MI x MI MFMA block) of accumulator value r of lane group g
+// (g = lane / MI; the column is lane % MI):
+//   16x16 FP32/INT32 accumulators: 4g + r;   16x16 FP64: g + 4r;
+//   32x32 (16 values per lane):    8*(r/4) + 4g + r%4.
+template <Elem E, int MI>
 __device__ __forceinline__ int out_row(int g, int r) {
-  if constexpr (E == Elem::F64)
+  if constexpr (MI == 32)
+    return 8 * (r >> 2) + 4 * g + (r & 3);
+  else if constexpr (E == Elem::F64)
     return g + 4 * r;
   else
     return 4 * g + r;
@@ -324,27 +334,141 @@ struct GemmParams {
   int tiles_m, tiles_n;
 };
 
+// LDS bytes used by one macro tile (two buffers of A and B).
+template <Elem E, int BM, int BN, int BK>
+constexpr int tile_lds_bytes() {
+  return 2 * (BM + BN) * (BK * (int)sizeof(typename ElemT<E>::T) + 16);
+}
+
+// One BM x BN output tile at (m0, n0) of C = alpha*A'*B' + beta*C, the whole
+// K range. Shared by the plain / batched kernel and the persistent grouped
+// kernel, so both produce bit-identical results for the same K step.
 template <Elem E, typename OutT, bool A_K, bool B_K, int BM, int BN, int BK,
-          int WM, int WN, bool VA, bool VB>
-__global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmParams p) {
+          int WM, int WN, bool VA, bool VB, int MI = 16>
+__device__ __forceinline__ void gemm_tile(
+    const typename ElemT<E>::T *__restrict__ A, int64_t lda,
+    const typename ElemT<E>::T *__restrict__ B, int64_t ldb, OutT *C,
+    int64_t ldc, int M, int N, int K, double alpha_d, double beta_d, int m0,
+    int n0, char *lds) {
   using Tr = ElemT<E>;
   using T = typename Tr::T;
   using Acc = typename Tr::Acc;
-  using AccV = typename Tr::AccV;
+  using AccV = typename AccOf<E, MI>::type;
   constexpr int VEC = Tr::VEC;
-  constexpr int CK = 4 * VEC;  // k per fragment chunk
+  constexpr int NG = 64 / MI;       // lane groups along k
+  constexpr int CK = NG * VEC;      // k per fragment chunk
+  constexpr int NVAL = MI * MI / 64;  // accumulator values per lane per block
   static_assert(BK % CK == 0, "BK must be a multiple of the chunk");
   static_assert(WM * WN == 4, "four waves");
   constexpr int WTM = BM / WM, WTN = BN / WN;
-  constexpr int FM = WTM / 16, FN = WTN / 16;
-  static_assert(FM * 16 == WTM && FN * 16 == WTN, "wave tile");
+  constexpr int FM = WTM / MI, FN = WTN / MI;
+  static_assert(FM * MI == WTM && FN * MI == WTN, "wave tile");
 
   using LA = TileLoader<T, BM, BK, A_K, VA>;
   using LB = TileLoader<T, BN, BK, B_K, VB>;
   constexpr int ROWB = LA::ROWB;
   constexpr int A_BYTES = BM * ROWB, B_BYTES = BN * ROWB;
-  __shared__ __attribute__((aligned(16))) char lds[2 * (A_BYTES + B_BYTES)];
 
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wm = wave / WN, wn = wave % WN;
+  const int lrow = lane % MI, lgrp = lane / MI;
+
+  AccV acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = AccV{};
+
+  LA la;
+  LB lb;
+  const int nk = (K + BK - 1) / BK;
+  la.load(A, lda, m0, 0, M, K);
+  lb.load(B, ldb, n0, 0, N, K);
+  la.store(lds);
+  lb.store(lds + A_BYTES);
+  __syncthreads();
+
+  for (int kt = 0; kt < nk; ++kt) {
+    char *cur = lds + (kt & 1) * (A_BYTES + B_BYTES);
+    char *nxt = lds + ((kt + 1) & 1) * (A_BYTES + B_BYTES);
+    const bool more = kt + 1 < nk;
+    if (more) {
+      la.load(A, lda, m0, (kt + 1) * BK, M, K);
+      lb.load(B, ldb, n0, (kt + 1) * BK, N, K);
+    }
+    const char *As = cur;
+    const char *Bs = cur + A_BYTES;
+#pragma unroll
+    for (int c = 0; c < BK / CK; ++c) {
+      uint4 af[FM], bf[FN];
+      const int kbyte = (c * CK + lgrp * VEC) * (int)sizeof(T);
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+        af[i] = *reinterpret_cast<const uint4 *>(
+            As + (wm * WTM + i * MI + lrow) * ROWB + kbyte);
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+        bf[j] = *reinterpret_cast<const uint4 *>(
+            Bs + (wn * WTN + j * MI + lrow) * ROWB + kbyte);
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+          mfma_step<E, MI, 0>(af[i], bf[j], acc[i][j]);
+      if constexpr (Tr::STEPS > 1) {
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j)
+            mfma_step<E, MI, 1>(af[i], bf[j], acc[i][j]);
+      }
+      if constexpr (Tr::STEPS > 2) {
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j)
+            mfma_step<E, MI, 2>(af[i], bf[j], acc[i][j]);
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j)
+            mfma_step<E, MI, 3>(af[i], bf[j], acc[i][j]);
+      }
+    }
+    if (more) {
+      la.store(nxt);
+      lb.store(nxt + A_BYTES);
+    }
+    __syncthreads();
+  }
+
+  // Epilogue: C = alpha*acc + beta*C; beta == 0 never reads C.
+  const Acc alpha = (Acc)alpha_d, beta = (Acc)beta_d;
+#pragma unroll
+  for (int i = 0; i < FM; ++i) {
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int col = n0 + wn * WTN + j * MI + lrow;
+      if (col >= N) continue;
+      const int rowb = m0 + wm * WTM + i * MI;
+#pragma unroll
+      for (int r = 0; r < NVAL; ++r) {
+        const int row = rowb + out_row<E, MI>(lgrp, r);
+        if (row >= M) continue;
+        OutT *dst = C + (int64_t)col * ldc + row;
+        Acc v = alpha * (Acc)acc[i][j][r];
+        if (beta != Acc(0)) v = fma_acc(beta, from_out<Acc>(*dst), v);
+        *dst = to_out<OutT>(v);
+      }
+    }
+  }
+}
+
+template <Elem E, typename OutT, bool A_K, bool B_K, int BM, int BN, int BK,
+          int WM, int WN, bool VA, bool VB, int MI = 16>
+__global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmParams p) {
+  using T = typename ElemT<E>::T;
+  __shared__ __attribute__((aligned(16))) char lds[tile_lds_bytes<E, BM, BN, BK>()];
   const int z = blockIdx.z;
   const T *A;
   const T *B;
@@ -361,103 +485,11 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmParams p) {
     C = static_cast<OutT *>(p.Cptrs[z]) + p.cOff;
   else
     C = static_cast<OutT *>(p.C) + p.cOff + (int64_t)z * p.strideC;
-
   int tile_m, tile_n;
   tile_coords(blockIdx.x, p.tiles_m, p.tiles_n, tile_m, tile_n);
-  const int m0 = tile_m * BM, n0 = tile_n * BN;
-
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int wm = wave / WN, wn = wave % WN;
-  const int lrow = lane & 15, lgrp = lane >> 4;
-
-  AccV acc[FM][FN];
-#pragma unroll
-  for (int i = 0; i < FM; ++i)
-#pragma unroll
-    for (int j = 0; j < FN; ++j) acc[i][j] = AccV{};
-
-  LA la;
-  LB lb;
-  const int nk = (p.K + BK - 1) / BK;
-  la.load(A, p.lda, m0, 0, p.M, p.K);
-  lb.load(B, p.ldb, n0, 0, p.N, p.K);
-  la.store(lds);
-  lb.store(lds + A_BYTES);
-  __syncthreads();
-
-  for (int kt = 0; kt < nk; ++kt) {
-    char *cur = lds + (kt & 1) * (A_BYTES + B_BYTES);
-    char *nxt = lds + ((kt + 1) & 1) * (A_BYTES + B_BYTES);
-    const bool more = kt + 1 < nk;
-    if (more) {
-      la.load(A, p.lda, m0, (kt + 1) * BK, p.M, p.K);
-      lb.load(B, p.ldb, n0, (kt + 1) * BK, p.N, p.K);
-    }
-    const char *As = cur;
-    const char *Bs = cur + A_BYTES;
-#pragma unroll
-    for (int c = 0; c < BK / CK; ++c) {
-      uint4 af[FM], bf[FN];
-      const int kbyte = (c * CK + lgrp * VEC) * (int)sizeof(T);
-#pragma unroll
-      for (int i = 0; i < FM; ++i)
-        af[i] = *reinterpret_cast<const uint4 *>(
-            As + (wm * WTM + i * 16 + lrow) * ROWB + kbyte);
-#pragma unroll
-      for (int j = 0; j < FN; ++j)
-        bf[j] = *reinterpret_cast<const uint4 *>(
-            Bs + (wn * WTN + j * 16 + lrow) * ROWB + kbyte);
-#pragma unroll
-      for (int i = 0; i < FM; ++i)
-#pragma unroll
-        for (int j = 0; j < FN; ++j) mfma_step<E, 0>(af[i], bf[j], acc[i][j]);
-      if constexpr (Tr::STEPS > 1) {
-#pragma unroll
-        for (int i = 0; i < FM; ++i)
-#pragma unroll
-          for (int j = 0; j < FN; ++j)
-            mfma_step<E, 1>(af[i], bf[j], acc[i][j]);
-      }
-      if constexpr (Tr::STEPS > 2) {
-#pragma unroll
-        for (int i = 0; i < FM; ++i)
-#pragma unroll
-          for (int j = 0; j < FN; ++j)
-            mfma_step<E, 2>(af[i], bf[j], acc[i][j]);
-#pragma unroll
-        for (int i = 0; i < FM; ++i)
-#pragma unroll
-          for (int j = 0; j < FN; ++j)
-            mfma_step<E, 3>(af[i], bf[j], acc[i][j]);
-      }
-    }
-    if (more) {
-      la.store(nxt);
-      lb.store(nxt + A_BYTES);
-    }
-    __syncthreads();
-  }
-
-  // Epilogue: C = alpha*acc + beta*C; beta == 0 never reads C.
-  const Acc alpha = (Acc)p.alpha, beta = (Acc)p.beta;
-#pragma unroll
-  for (int i = 0; i < FM; ++i) {
-#pragma unroll
-    for (int j = 0; j < FN; ++j) {
-      const int col = n0 + wn * WTN + j * 16 + lrow;
-      if (col >= p.N) continue;
-      const int rowb = m0 + wm * WTM + i * 16;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = rowb + out_row<E>(lgrp, r);
-        if (row >= p.M) continue;
-        OutT *dst = C + (int64_t)col * p.ldc + row;
-        Acc v = alpha * (Acc)acc[i][j][r];
-        if (beta != Acc(0)) v = fma_acc(beta, from_out<Acc>(*dst), v);
-        *dst = to_out<OutT>(v);
-      }
-    }
-  }
+  gemm_tile<E, OutT, A_K, B_K, BM, BN, BK, WM, WN, VA, VB, MI>(
+      A, p.lda, B, p.ldb, C, p.ldc, p.M, p.N, p.K, p.alpha, p.beta,
+      tile_m * BM, tile_n * BN, lds);
 }
 
 }  // namespace hcb

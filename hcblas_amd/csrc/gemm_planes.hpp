@@ -44,18 +44,21 @@ struct PlaneParams {
   int tiles_m, tiles_n;
 };
 
-template <Elem E, typename OutT, int BM, int BN, int BK, int WM, int WN>
+template <Elem E, typename OutT, int BM, int BN, int BK, int WM, int WN,
+          int MI = 16>
 __global__ __launch_bounds__(256) void gemm_planes_kernel(PlaneParams p) {
   using Tr = ElemT<E>;
   using T = typename Tr::T;
-  using AccV = typename Tr::AccV;
+  using AccV = typename AccOf<E, MI>::type;
   constexpr bool INT = E == Elem::I8;
   constexpr int VEC = Tr::VEC;
-  constexpr int CK = 4 * VEC;
+  constexpr int NG = 64 / MI;
+  constexpr int CK = NG * VEC;
+  constexpr int NVAL = MI * MI / 64;
   static_assert(Tr::STEPS == 1, "packed types only");
   static_assert(BK % CK == 0, "BK");
   constexpr int WTM = BM / WM, WTN = BN / WN;
-  constexpr int FM = WTM / 16, FN = WTN / 16;
+  constexpr int FM = WTM / MI, FN = WTN / MI;
   using LA = TileLoader<T, BM, BK, true, true>;
   using LB = TileLoader<T, BN, BK, true, true>;
   constexpr int ROWB = LA::ROWB;
@@ -67,13 +70,13 @@ __global__ __launch_bounds__(256) void gemm_planes_kernel(PlaneParams p) {
   const int m0 = tile_m * BM, n0 = tile_n * BN;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int wm = wave / WN, wn = wave % WN;
-  const int lrow = lane & 15, lgrp = lane >> 4;
+  const int lrow = lane % MI, lgrp = lane / MI;
   const int64_t ld = (int64_t)p.S * p.Kp;
   const T *Ap = static_cast<const T *>(p.Ap);
   const T *Bp = static_cast<const T *>(p.Bp);
 
   AccV acc[FM][FN];
-  double facc[INT ? FM : 1][INT ? FN : 1][INT ? 4 : 1];
+  double facc[INT ? FM : 1][INT ? FN : 1][INT ? NVAL : 1];
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -84,7 +87,7 @@ __global__ __launch_bounds__(256) void gemm_planes_kernel(PlaneParams p) {
 #pragma unroll
       for (int j = 0; j < FN; ++j)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) facc[i][j][r] = 0.0;
+        for (int r = 0; r < NVAL; ++r) facc[i][j][r] = 0.0;
   }
 
   const int kt_per_plane = (int)(p.Kp / BK);
@@ -130,15 +133,16 @@ __global__ __launch_bounds__(256) void gemm_planes_kernel(PlaneParams p) {
 #pragma unroll
       for (int i = 0; i < FM; ++i)
         af[i] = *reinterpret_cast<const uint4 *>(
-            As + (wm * WTM + i * 16 + lrow) * ROWB + kbyte);
+            As + (wm * WTM + i * MI + lrow) * ROWB + kbyte);
 #pragma unroll
       for (int j = 0; j < FN; ++j)
         bf[j] = *reinterpret_cast<const uint4 *>(
-            Bs + (wn * WTN + j * 16 + lrow) * ROWB + kbyte);
+            Bs + (wn * WTN + j * MI + lrow) * ROWB + kbyte);
 #pragma unroll
       for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int j = 0; j < FN; ++j) mfma_step<E, 0>(af[i], bf[j], acc[i][j]);
+        for (int j = 0; j < FN; ++j)
+          mfma_step<E, MI, 0>(af[i], bf[j], acc[i][j]);
     }
     if (more) {
       la.store(nxt);
@@ -153,7 +157,7 @@ __global__ __launch_bounds__(256) void gemm_planes_kernel(PlaneParams p) {
 #pragma unroll
           for (int j = 0; j < FN; ++j) {
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
+            for (int r = 0; r < NVAL; ++r)
               facc[i][j][r] = __builtin_fma((double)acc[i][j][r], w,
                                             facc[i][j][r]);
             acc[i][j] = AccV{};
@@ -170,14 +174,14 @@ __global__ __launch_bounds__(256) void gemm_planes_kernel(PlaneParams p) {
   for (int i = 0; i < FM; ++i) {
 #pragma unroll
     for (int j = 0; j < FN; ++j) {
-      const int col = n0 + wn * WTN + j * 16 + lrow;
+      const int col = n0 + wn * WTN + j * MI + lrow;
       if (col >= p.N) continue;
-      const int rowb = m0 + wm * WTM + i * 16 + lgrp * 4;
+      const int rowb = m0 + wm * WTM + i * MI;
       int ce = 0;
       if constexpr (INT) ce = p.col_exp[col];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = rowb + r;
+      for (int r = 0; r < NVAL; ++r) {
+        const int row = rowb + out_row<E, MI>(lgrp, r);
         if (row >= p.M) continue;
         OutT *dst = C + (int64_t)col * p.ldc + row;
         if constexpr (INT) {

@@ -484,6 +484,52 @@ hcblasStatus_t hcblas_dgemm_ozaki_int8(hcblasHandle_t handle,
                                        int64_t bOffset, int64_t cOffset);
 
 /* ------------------------------------------------------------------------ */
+/* Grouped GEMM (no reference counterpart; the reference's batched GEMM     */
+/* takes one shape for the whole batch, sgemm_array_kernels_batch.cpp)      */
+/* ------------------------------------------------------------------------ */
+
+/* One problem of a group: C = alpha*op(A)*op(B) + beta*C with its own dtype
+ * (HCBLAS_R_32F, or HCBLAS_R_16F = binary16 A/B/C with fp32 accumulation),
+ * storage order, transposes (N or T), sizes and leading dimensions. */
+typedef struct hcblasGemmProblem {
+  int32_t dtype;
+  int32_t order;
+  int32_t transA;
+  int32_t transB;
+  int32_t M, N, K;
+  int32_t reserved;
+  float alpha, beta;
+  const void *A;
+  int64_t lda;
+  const void *B;
+  int64_t ldb;
+  void *C;
+  int64_t ldc;
+} hcblasGemmProblem;
+
+typedef struct hcblasGroupedPlan *hcblasGroupedPlan_t;
+
+/* Validates a HOST problem table, copies it to the device in canonical form,
+ * builds the tile work list and captures the persistent launch in a hipGraph.
+ * The device pointers in the table must stay valid while the plan is used. */
+hcblasStatus_t hcblasGroupedPlanCreate(hcblasHandle_t handle,
+                                       const hcblasGemmProblem *problems,
+                                       int count, hcblasGroupedPlan_t *plan);
+/* Times every (tile shape, work order) candidate `reps` times on the plan's
+ * own problems -- which executes them, so C is updated each time -- and
+ * keeps the fastest. */
+hcblasStatus_t hcblasGroupedPlanAutotune(hcblasHandle_t handle,
+                                         hcblasGroupedPlan_t plan, int reps);
+/* Runs every problem of the plan: one graph launch on the handle's stream. */
+hcblasStatus_t hcblasGroupedGemm(hcblasHandle_t handle,
+                                 hcblasGroupedPlan_t plan);
+/* Tile edge, work order (0 longest first, 1 table order), number of tiles and
+ * persistent workgroups of the plan's current schedule. */
+hcblasStatus_t hcblasGroupedPlanInfo(hcblasGroupedPlan_t plan, int *tile,
+                                     int *order, int *tiles, int *workgroups);
+hcblasStatus_t hcblasGroupedPlanDestroy(hcblasGroupedPlan_t plan);
+
+/* ------------------------------------------------------------------------ */
 /* Library information                                                      */
 /* ------------------------------------------------------------------------ */
 

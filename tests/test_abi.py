@@ -60,7 +60,8 @@ def test_library_exports_every_declared_symbol():
 def test_null_handle_is_not_initialized():
     lib = _lib.load()
     protos = _abi.parse_header()
-    skip = {"hcblasCreate", "hcblasDestroy", "hcblasGetVersionString"}
+    skip = {"hcblasCreate", "hcblasDestroy", "hcblasGetVersionString",
+            "hcblasGroupedPlanInfo", "hcblasGroupedPlanDestroy"}
     checked = 0
     for name, proto in protos.items():
         if name in skip:
@@ -74,6 +75,12 @@ def test_null_handle_is_not_initialized():
         assert st == 1, f"{name}: expected NOT_INITIALIZED, got {st}"
         checked += 1
     assert checked > 60
+
+
+def test_plan_entries_reject_null_plan():
+    lib = _lib.load()
+    assert lib.hcblasGroupedPlanInfo(None, None, None, None, None) == 3
+    assert lib.hcblasGroupedPlanDestroy(None) == 3
 
 
 def test_destroy_null_and_double_destroy_contract():
