@@ -98,31 +98,54 @@ def bandwidth_ops(hc, h, torch):
     return ops
 
 
+def measured_traffic(op):
+    """HBM bytes per launch of `op`'s kernel from profiles/traffic.json, which
+    holds rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE per dispatch of the bench
+    kernels, each scaled by what the counter reads for scopy (a plain copy of
+    known size). None when no such measurement is committed."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)["ops"].get(op)
+    except (OSError, ValueError, KeyError):
+        return None
+    if not rec:
+        return None
+    return int(rec["fetch_bytes"] + rec["write_bytes"])
+
+
 def time_bandwidth(torch, ops, steps, warmup):
+    """Whole-step time (every op once per step, including the host round trip
+    of the synchronous dot / asum results), then each op's own duration from
+    a burst of `steps` back-to-back calls bracketed by events on the stream
+    the library launches on (torch's current stream, set on the handle), so
+    host launch gaps do not count against a kernel."""
     for _ in range(warmup):
         for _, _, fn in ops:
             fn()
     torch.cuda.synchronize()
-    ev = [(torch.cuda.Event(enable_timing=True),
-           torch.cuda.Event(enable_timing=True)) for _ in ops]
-    per_op = [0.0] * len(ops)
     t0 = time.perf_counter()
     start = torch.cuda.Event(enable_timing=True)
     end = torch.cuda.Event(enable_timing=True)
     start.record()
     for _ in range(steps):
-        for i, (_, _, fn) in enumerate(ops):
-            ev[i][0].record()
+        for _, _, fn in ops:
             fn()
-            ev[i][1].record()
-        torch.cuda.synchronize()
-        for i in range(len(ops)):
-            per_op[i] += ev[i][0].elapsed_time(ev[i][1])
     end.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     step_ms = start.elapsed_time(end) / steps
-    per_op = [t / steps for t in per_op]
+    per_op = []
+    for _, _, fn in ops:
+        fn()
+        s = torch.cuda.Event(enable_timing=True)
+        e = torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(steps):
+            fn()
+        e.record()
+        torch.cuda.synchronize()
+        per_op.append(s.elapsed_time(e) / steps)
     return step_ms, per_op, wall
 
 
@@ -187,6 +210,10 @@ def gemm_section(hc, h, torch, oracle, n, reps=5):
             oz = api.hcblas_dgemm_ozaki_int8
             run("dgemm_ozaki_int8", dt,
                 lambda: oz(h, 1, 0, 0, n, n, n, 1.0, A, n, B, n, 0.0, C, n, 0, 0, 0),
+                A, B, C, pk)
+            crt = api.hcblas_dgemm_ozaki_crt
+            run("dgemm_ozaki_crt", dt,
+                lambda: crt(h, 1, 0, 0, n, n, n, 1.0, A, n, B, n, 0.0, C, n, 0, 0, 0),
                 A, B, C, pk)
         del A, B, C
         torch.cuda.empty_cache()
@@ -348,18 +375,20 @@ def cpu_baseline(oracle):
         (16 * n_d, lambda: L.orc_ddot(n_d, P(dx), 1, P(dy), 1)),
         (8 * n_d, lambda: L.orc_dasum(n_d, P(dx), 1)),
     ]
-    total_b, total_t = 0, 0.0
-    for reps in range(2):
+    total_b, total_t, passes = 0, 0.0, 0
+    # whole passes over the sample until about 12 s of CPU work
+    while passes < 2 or total_t < 12.0:
         for b, fn in jobs:
             t0 = time.perf_counter()
             fn()
             total_t += time.perf_counter() - t0
             total_b += b
+        passes += 1
     del np
     return {"value": round(total_b / total_t / 1e9, 3), "unit": "GB/s", "cores": 1,
             "kind": "port",
-            "sample": "bandwidth set at reduced size, 2 passes: S/DGEMV N+T and "
-                      "S/DGER at 8192^2, SAXPY/SSCAL/SCOPY/SDOT/SASUM n=2^25, "
+            "sample": f"bandwidth set at reduced size, {passes} passes: S/DGEMV N+T "
+                      "and S/DGER at 8192^2, SAXPY/SSCAL/SCOPY/SDOT/SASUM n=2^25, "
                       "DDOT/DASUM n=2^24 (oracle/refblas.c, netlib loop order)",
             "seconds": round(total_t, 2)}
 
@@ -385,7 +414,8 @@ def main():
     achieved = dbytes / (per_op[dom] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "kernel": dname, "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(dname),
                 "avg_ms": round(per_op[dom], 4)}
     result = {
         "metric": "GEMM TFLOP/s split vs plain at equal err vs FP64; GEMV/dot HBM GB/s",

@@ -40,6 +40,7 @@ struct hcblasContext {
   hipStream_t stream = nullptr;
   int math_mode = HCBLAS_DEFAULT_MATH;
   int ozaki_slices = 8;
+  int ozaki_moduli = 16;
 
   // dot / asum: per-workgroup partials, a ticket counter that the last
   // workgroup resets to zero, the device result and a pinned host mirror.
@@ -60,6 +61,8 @@ struct hcblasContext {
   // apart from gemm_ws: a split complex GEMM reads the embedding while it
   // writes the planes).
   hcb::Workspace split_ws;
+  // FP32 copies of BF16 operands for the adjoint GEMMs.
+  hcb::Workspace adj_ws;
 };
 
 namespace hcb {

@@ -437,6 +437,9 @@ hcblasStatus_t gemm_complex(hcblasHandle_t h, const Canon &c, T ar, T ai,
       return gemm_split_bf16(h, r, 1.f, (float)br, static_cast<float *>(C),
                              2 * ldc, 2 * cOff, 2 * strideC, Cptrs, batch);
   } else {
+    if (h->math_mode & HCBLAS_OZAKI_CRT_MATH)
+      return gemm_ozaki_crt(h, r, 1.0, (double)br, static_cast<double *>(C),
+                            2 * ldc, 2 * cOff, 2 * strideC, Cptrs, batch);
     if (h->math_mode & HCBLAS_OZAKI_INT8_MATH)
       return gemm_ozaki_int8(h, r, 1.0, (double)br, static_cast<double *>(C),
                              2 * ldc, 2 * cOff, 2 * strideC, Cptrs, batch);
@@ -459,6 +462,58 @@ template hcblasStatus_t gemm_real<Elem::F64, double>(hcblasHandle_t,
 }  // namespace hcb
 
 // ---------------------------------------------------------------------------
+// Adjoint helpers
+// ---------------------------------------------------------------------------
+namespace hcb {
+namespace {
+
+// dst[i] = float(src[i]) for bfloat16 bit patterns (exact).
+__global__ __launch_bounds__(256) void bf16_to_f32_kernel(int64_t n,
+                                                          const uint16_t *src,
+                                                          float *dst) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * 256)
+    dst[i] = __uint_as_float((uint32_t)src[i] << 16);
+}
+
+// dst = beta * src over a batch of rows x cols matrices stored with
+// (element (i, j) at i * rs + j * cs); beta == 0 writes zeros.
+template <typename T>
+__global__ __launch_bounds__(256) void scaled_copy_kernel(
+    int rows, int cols, const T *src, int64_t src_rs, int64_t src_cs,
+    int64_t src_stride, T *dst, int64_t dst_rs, int64_t dst_cs,
+    int64_t dst_stride, T beta) {
+  const T *s = src + (int64_t)blockIdx.z * src_stride;
+  T *d = dst + (int64_t)blockIdx.z * dst_stride;
+  const int64_t total = (int64_t)rows * cols;
+  for (int64_t f = (int64_t)blockIdx.x * 256 + threadIdx.x; f < total;
+       f += (int64_t)gridDim.x * 256) {
+    int64_t i, j;
+    if (dst_rs == 1) {  // column-major: i fastest
+      j = f / rows;
+      i = f - j * rows;
+    } else {
+      i = f / cols;
+      j = f - i * cols;
+    }
+    d[i * dst_rs + j * dst_cs] = beta == T(0) ? T(0) : beta * s[i * src_rs + j * src_cs];
+  }
+}
+
+// Elements spanned by a strided batch of rows x cols matrices.
+int64_t batch_span(hcblasOrder_t order, int rows, int cols, int64_t ld,
+                   int64_t stride, int batch) {
+  if (rows == 0 || cols == 0 || batch == 0) return 0;
+  const int64_t foot = order == HCBLAS_COL_MAJOR
+                           ? ld * (cols - 1) + rows
+                           : ld * (rows - 1) + cols;
+  return (int64_t)(batch - 1) * stride + foot;
+}
+
+}  // namespace
+}  // namespace hcb
+
+// ---------------------------------------------------------------------------
 // Entry points
 // ---------------------------------------------------------------------------
 using namespace hcb;
@@ -476,13 +531,12 @@ inline float half_to_float(hcblasHalf hb) {
 hcblasStatus_t dispatch(hcblasHandle_t h, hcblasDatatype_t dt, const Canon &c,
                         const void *alpha, const void *beta, void *C,
                         int64_t ldc, int64_t cOff, int64_t strideC,
-                        void *const *Cptrs, int64_t batch, bool force_split,
-                        bool force_ozaki) {
+                        void *const *Cptrs, int64_t batch, int mode) {
   switch (dt) {
     case HCBLAS_R_32F: {
       const float a = *static_cast<const float *>(alpha);
       const float b = *static_cast<const float *>(beta);
-      if (force_split || (h->math_mode & HCBLAS_SPLIT_BF16_MATH))
+      if (mode & HCBLAS_SPLIT_BF16_MATH)
         return gemm_split_bf16(h, c, a, b, static_cast<float *>(C), ldc, cOff,
                                strideC, Cptrs, batch);
       return gemm_real<Elem::F32, float>(h, c, a, b, C, ldc, cOff, strideC,
@@ -491,7 +545,10 @@ hcblasStatus_t dispatch(hcblasHandle_t h, hcblasDatatype_t dt, const Canon &c,
     case HCBLAS_R_64F: {
       const double a = *static_cast<const double *>(alpha);
       const double b = *static_cast<const double *>(beta);
-      if (force_ozaki || (h->math_mode & HCBLAS_OZAKI_INT8_MATH))
+      if (mode & HCBLAS_OZAKI_CRT_MATH)
+        return gemm_ozaki_crt(h, c, a, b, static_cast<double *>(C), ldc, cOff,
+                              strideC, Cptrs, batch);
+      if (mode & HCBLAS_OZAKI_INT8_MATH)
         return gemm_ozaki_int8(h, c, a, b, static_cast<double *>(C), ldc,
                                cOff, strideC, Cptrs, batch);
       return gemm_real<Elem::F64, double>(h, c, a, b, C, ldc, cOff, strideC,
@@ -538,8 +595,7 @@ hcblasStatus_t gemm_common(hcblasHandle_t h, hcblasDatatype_t dt,
                            int64_t sB, const void *const *Bp, const void *beta,
                            void *C, int64_t ldc, int64_t cOff, int64_t sC,
                            void *const *Cp, int64_t batch,
-                           bool force_split = false,
-                           bool force_ozaki = false) {
+                           int force_mode = 0) {
   if (!handle_ok(h)) return HCBLAS_STATUS_NOT_INITIALIZED;
   if (!valid_dtype(dt) || batch < 0) return HCBLAS_STATUS_INVALID_VALUE;
   Canon c{};
@@ -551,7 +607,7 @@ hcblasStatus_t gemm_common(hcblasHandle_t h, hcblasDatatype_t dt,
   if (hipSetDevice(h->device) != hipSuccess)
     return HCBLAS_STATUS_EXECUTION_FAILED;
   return dispatch(h, dt, c, alpha, beta, C, ldc, cOff, sC, Cp, batch,
-                  force_split, force_ozaki);
+                  force_mode ? force_mode : h->math_mode);
 }
 
 }  // namespace
@@ -745,7 +801,8 @@ hcblasStatus_t hcblas_sgemm_split_bf16(
     float *C, int64_t ldc, int64_t aOffset, int64_t bOffset, int64_t cOffset) {
   return gemm_common(handle, HCBLAS_R_32F, order, typeA, typeB, M, N, K, alpha,
                      A, lda, aOffset, 0, nullptr, B, ldb, bOffset, 0, nullptr,
-                     beta, C, ldc, cOffset, 0, nullptr, 1, true, false);
+                     beta, C, ldc, cOffset, 0, nullptr, 1,
+                     HCBLAS_SPLIT_BF16_MATH);
 }
 hcblasStatus_t hcblas_dgemm_ozaki_int8(
     hcblasHandle_t handle, hcblasOrder_t order, hcblasOperation_t typeA,
@@ -755,7 +812,141 @@ hcblasStatus_t hcblas_dgemm_ozaki_int8(
     int64_t bOffset, int64_t cOffset) {
   return gemm_common(handle, HCBLAS_R_64F, order, typeA, typeB, M, N, K, alpha,
                      A, lda, aOffset, 0, nullptr, B, ldb, bOffset, 0, nullptr,
-                     beta, C, ldc, cOffset, 0, nullptr, 1, false, true);
+                     beta, C, ldc, cOffset, 0, nullptr, 1,
+                     HCBLAS_OZAKI_INT8_MATH);
+}
+hcblasStatus_t hcblas_dgemm_ozaki_crt(
+    hcblasHandle_t handle, hcblasOrder_t order, hcblasOperation_t typeA,
+    hcblasOperation_t typeB, int M, int N, int K, const double *alpha,
+    const double *A, int64_t lda, const double *B, int64_t ldb,
+    const double *beta, double *C, int64_t ldc, int64_t aOffset,
+    int64_t bOffset, int64_t cOffset) {
+  return gemm_common(handle, HCBLAS_R_64F, order, typeA, typeB, M, N, K, alpha,
+                     A, lda, aOffset, 0, nullptr, B, ldb, bOffset, 0, nullptr,
+                     beta, C, ldc, cOffset, 0, nullptr, 1,
+                     HCBLAS_OZAKI_CRT_MATH);
+}
+
+// Adjoints of the strided batched GEMM C = alpha*op(A)*op(B) + beta*C_in
+// for a cotangent dC (no reference counterpart):
+//   dA = alpha * dC * op(B)^T   (stored like A: alpha * op(B) * dC^T when
+//                                op(A) = A^T),
+//   dB = alpha * op(A)^T * dC   (alpha * dC^T * op(A) when op(B) = B^T),
+//   dC_in = beta * dC.
+// Each product runs on the MFMA GEMM kernels with the adjoint orientation;
+// BF16 operands are widened to FP32 once (exactly) and the adjoints are
+// computed and returned in FP32, the accumulation type of the forward.
+hcblasStatus_t hcblas_gemm_strided_batched_backward(
+    hcblasHandle_t handle, hcblasDatatype_t dtype, hcblasOrder_t order,
+    hcblasOperation_t typeA, hcblasOperation_t typeB, int M, int N, int K,
+    const void *alpha, const void *beta, const void *A, int64_t lda,
+    int64_t strideA, const void *B, int64_t ldb, int64_t strideB,
+    const void *dC, int64_t lddc, int64_t strideDC, void *dA, int64_t ldda,
+    int64_t strideDA, void *dB, int64_t lddb, int64_t strideDB, void *dCin,
+    int64_t lddcin, int64_t strideDCin, int batchCount) {
+  if (!handle_ok(handle)) return HCBLAS_STATUS_NOT_INITIALIZED;
+  if (dtype != HCBLAS_R_32F && dtype != HCBLAS_R_64F && dtype != HCBLAS_R_16BF)
+    return HCBLAS_STATUS_INVALID_VALUE;
+  if (order != HCBLAS_ROW_MAJOR && order != HCBLAS_COL_MAJOR)
+    return HCBLAS_STATUS_INVALID_VALUE;
+  if ((unsigned)typeA > 2u || (unsigned)typeB > 2u)
+    return HCBLAS_STATUS_INVALID_VALUE;
+  if (M < 0 || N < 0 || K < 0 || batchCount < 0)
+    return HCBLAS_STATUS_INVALID_VALUE;
+  if (alpha == nullptr || beta == nullptr || dC == nullptr)
+    return HCBLAS_STATUS_INVALID_VALUE;
+  if (batchCount == 0) return HCBLAS_STATUS_SUCCESS;
+  const bool ta = typeA != HCBLAS_OP_N, tb = typeB != HCBLAS_OP_N;
+  const hcblasOperation_t OP_N = HCBLAS_OP_N, OP_T = HCBLAS_OP_T;
+  const hcblasDatatype_t wt = dtype == HCBLAS_R_16BF ? HCBLAS_R_32F : dtype;
+  const float zf = 0.f;
+  const double zd = 0.0;
+  const void *zero = wt == HCBLAS_R_64F ? (const void *)&zd : (const void *)&zf;
+  const void *Aw = A, *Bw = B;
+  if (dtype == HCBLAS_R_16BF && (dA || dB)) {
+    // A is rows_a x cols_a in storage, B rows_b x cols_b.
+    const int ra = ta ? K : M, ca = ta ? M : K;
+    const int rb = tb ? N : K, cb = tb ? K : N;
+    const int64_t na = dB ? batch_span(order, ra, ca, lda, strideA, batchCount) : 0;
+    const int64_t nb = dA ? batch_span(order, rb, cb, ldb, strideB, batchCount) : 0;
+    if ((na && A == nullptr) || (nb && B == nullptr))
+      return HCBLAS_STATUS_INVALID_VALUE;
+    if (ensure_workspace(handle->adj_ws, sizeof(float) * (size_t)(na + nb) + 256) !=
+        hipSuccess)
+      return HCBLAS_STATUS_ALLOC_FAILED;
+    float *wa = static_cast<float *>(handle->adj_ws.ptr);
+    float *wb = wa + na;
+    if (hipSetDevice(handle->device) != hipSuccess)
+      return HCBLAS_STATUS_EXECUTION_FAILED;
+    if (na) {
+      bf16_to_f32_kernel<<<grid_for(na, handle->num_cus), 256, 0,
+                           handle->stream>>>(
+          na, static_cast<const uint16_t *>(A), wa);
+      Aw = wa;
+    }
+    if (nb) {
+      bf16_to_f32_kernel<<<grid_for(nb, handle->num_cus), 256, 0,
+                           handle->stream>>>(
+          nb, static_cast<const uint16_t *>(B), wb);
+      Bw = wb;
+    }
+    hcblasStatus_t st = launch_status();
+    if (st != HCBLAS_STATUS_SUCCESS) return st;
+  }
+  hcblasStatus_t st = HCBLAS_STATUS_SUCCESS;
+  if (dA) {
+    if (!ta)
+      st = gemm_common(handle, wt, order, OP_N, tb ? OP_N : OP_T, M, K, N,
+                       alpha, dC, lddc, 0, strideDC, nullptr, Bw, ldb, 0,
+                       strideB, nullptr, zero, dA, ldda, 0, strideDA, nullptr,
+                       batchCount);
+    else
+      st = gemm_common(handle, wt, order, tb ? OP_T : OP_N, OP_T, K, M, N,
+                       alpha, Bw, ldb, 0, strideB, nullptr, dC, lddc, 0,
+                       strideDC, nullptr, zero, dA, ldda, 0, strideDA, nullptr,
+                       batchCount);
+    if (st != HCBLAS_STATUS_SUCCESS) return st;
+  }
+  if (dB) {
+    if (!tb)
+      st = gemm_common(handle, wt, order, ta ? OP_N : OP_T, OP_N, K, N, M,
+                       alpha, Aw, lda, 0, strideA, nullptr, dC, lddc, 0,
+                       strideDC, nullptr, zero, dB, lddb, 0, strideDB, nullptr,
+                       batchCount);
+    else
+      st = gemm_common(handle, wt, order, OP_T, ta ? OP_T : OP_N, N, K, M,
+                       alpha, dC, lddc, 0, strideDC, nullptr, Aw, lda, 0,
+                       strideA, nullptr, zero, dB, lddb, 0, strideDB, nullptr,
+                       batchCount);
+    if (st != HCBLAS_STATUS_SUCCESS) return st;
+  }
+  if (dCin && M > 0 && N > 0) {
+    const bool col = order == HCBLAS_COL_MAJOR;
+    const int64_t src_ld = lddc, dst_ld = lddcin;
+    if (src_ld < (col ? M : N) || dst_ld < (col ? M : N))
+      return HCBLAS_STATUS_INVALID_VALUE;
+    if (hipSetDevice(handle->device) != hipSuccess)
+      return HCBLAS_STATUS_EXECUTION_FAILED;
+    for (int64_t b0 = 0; b0 < batchCount; b0 += 65535) {
+      const unsigned nb = (unsigned)std::min<int64_t>(65535, batchCount - b0);
+      dim3 g(grid_for((int64_t)M * N, handle->num_cus), 1, nb);
+      if (wt == HCBLAS_R_64F)
+        scaled_copy_kernel<double><<<g, 256, 0, handle->stream>>>(
+            M, N, static_cast<const double *>(dC) + b0 * strideDC,
+            col ? 1 : src_ld, col ? src_ld : 1, strideDC,
+            static_cast<double *>(dCin) + b0 * strideDCin, col ? 1 : dst_ld,
+            col ? dst_ld : 1, strideDCin, *static_cast<const double *>(beta));
+      else
+        scaled_copy_kernel<float><<<g, 256, 0, handle->stream>>>(
+            M, N, static_cast<const float *>(dC) + b0 * strideDC,
+            col ? 1 : src_ld, col ? src_ld : 1, strideDC,
+            static_cast<float *>(dCin) + b0 * strideDCin, col ? 1 : dst_ld,
+            col ? dst_ld : 1, strideDCin, *static_cast<const float *>(beta));
+      st = launch_status();
+      if (st != HCBLAS_STATUS_SUCCESS) return st;
+    }
+  }
+  return HCBLAS_STATUS_SUCCESS;
 }
 
 }  // extern "C"

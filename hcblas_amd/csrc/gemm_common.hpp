@@ -33,5 +33,9 @@ hcblasStatus_t gemm_ozaki_int8(hcblasHandle_t h, const Canon &c, double alpha,
                                double beta, double *C, int64_t ldc,
                                int64_t cOff, int64_t strideC,
                                void *const *Cptrs, int64_t batch);
+hcblasStatus_t gemm_ozaki_crt(hcblasHandle_t h, const Canon &c, double alpha,
+                              double beta, double *C, int64_t ldc,
+                              int64_t cOff, int64_t strideC,
+                              void *const *Cptrs, int64_t batch);
 
 }  // namespace hcb

@@ -11,7 +11,7 @@
 // B[n + k*ldb] otherwise.
 //
 // Tiling:
-//   - workgroup = 256 threads = 4 waves in a WM x WN grid, macro tile BM x BN,
+//   - workgroup = 64*WM*WN threads (WM x WN waves), macro tile BM x BN,
 //     K step BK; each wave owns a (BM/WM) x (BN/WN) sub-tile made of 16x16
 //     MFMA blocks (v_mfma_*_16x16x{4,4,32,32,64}).
 //   - LDS holds both operands k-contiguous ([rows][BK] + 16 bytes of padding
@@ -175,11 +175,11 @@ __device__ __forceinline__ void mfma_step(const uint4 &a, const uint4 &b,
 // KCONT: the global operand is k-contiguous (element (r,k) at P[k + r*ld]).
 // VLOAD: 16-byte global loads are legal (base, ld aligned to the vector).
 // ---------------------------------------------------------------------------
-template <typename T, int ROWS, int BK, bool KCONT, bool VLOAD>
+template <typename T, int ROWS, int BK, bool KCONT, bool VLOAD, int NTH = 256>
 struct TileLoader {
   static constexpr int VEC = 16 / (int)sizeof(T);
   static constexpr int NV = ROWS * BK / VEC;  // 16-byte vectors per tile
-  static constexpr int PER_THREAD = (NV + 255) / 256;
+  static constexpr int PER_THREAD = (NV + NTH - 1) / NTH;
   static constexpr int ROWB = BK * (int)sizeof(T) + 16;  // LDS row stride
   // Transposing (r-contiguous) tiles: each group of 64 vectors covers
   // 8 r-vectors x 8 k, so a wave reads 128-byte runs of global memory and its
@@ -209,8 +209,8 @@ struct TileLoader {
                                        int r0, int k0, int R, int K) {
 #pragma unroll
     for (int i = 0; i < PER_THREAD; ++i) {
-      const int v = threadIdx.x + i * 256;
-      if (NV % 256 != 0 && v >= NV) continue;
+      const int v = threadIdx.x + i * NTH;
+      if (NV % NTH != 0 && v >= NV) continue;
       int r, k;
       vec_pos(v, r, k);
       const int gr = r0 + r, gk = k0 + k;
@@ -243,8 +243,8 @@ struct TileLoader {
   __device__ __forceinline__ void store(char *lds) const {
 #pragma unroll
     for (int i = 0; i < PER_THREAD; ++i) {
-      const int v = threadIdx.x + i * 256;
-      if (NV % 256 != 0 && v >= NV) continue;
+      const int v = threadIdx.x + i * NTH;
+      if (NV % NTH != 0 && v >= NV) continue;
       int r, k;
       vec_pos(v, r, k);
       if (KCONT) {
@@ -340,32 +340,42 @@ constexpr int tile_lds_bytes() {
   return 2 * (BM + BN) * (BK * (int)sizeof(typename ElemT<E>::T) + 16);
 }
 
-// One BM x BN output tile at (m0, n0) of C = alpha*A'*B' + beta*C, the whole
-// K range. Shared by the plain / batched kernel and the persistent grouped
-// kernel, so both produce bit-identical results for the same K step.
-template <Elem E, typename OutT, bool A_K, bool B_K, int BM, int BN, int BK,
-          int WM, int WN, bool VA, bool VB, int MI = 16>
-__device__ __forceinline__ void gemm_tile(
+// Wave sub-tile geometry of a BM x BN macro tile split over WM x WN waves.
+template <Elem E, int BM, int BN, int WM, int WN, int MI>
+struct WaveTile {
+  static constexpr int WTM = BM / WM, WTN = BN / WN;
+  static constexpr int FM = WTM / MI, FN = WTN / MI;
+  static constexpr int NVAL = MI * MI / 64;  // accumulator values per block
+  static_assert(FM * MI == WTM && FN * MI == WTN, "wave tile");
+  using AccV = typename AccOf<E, MI>::type;
+};
+
+// Main loop of one BM x BN tile at (m0, n0): acc = A'(m0.., :) * B'(:, n0..)
+// over the whole K range. Shared by the plain / batched kernel, the grouped
+// kernel and the residue kernel, so all of them produce bit-identical sums
+// for the same K step.
+template <Elem E, bool A_K, bool B_K, int BM, int BN, int BK, int WM, int WN,
+          bool VA, bool VB, int MI>
+__device__ __forceinline__ void gemm_mainloop(
     const typename ElemT<E>::T *__restrict__ A, int64_t lda,
-    const typename ElemT<E>::T *__restrict__ B, int64_t ldb, OutT *C,
-    int64_t ldc, int M, int N, int K, double alpha_d, double beta_d, int m0,
-    int n0, char *lds) {
+    const typename ElemT<E>::T *__restrict__ B, int64_t ldb, int M, int N,
+    int K, int m0, int n0, char *lds,
+    typename WaveTile<E, BM, BN, WM, WN, MI>::AccV (
+        &acc)[WaveTile<E, BM, BN, WM, WN, MI>::FM]
+             [WaveTile<E, BM, BN, WM, WN, MI>::FN]) {
   using Tr = ElemT<E>;
   using T = typename Tr::T;
-  using Acc = typename Tr::Acc;
-  using AccV = typename AccOf<E, MI>::type;
+  using WT = WaveTile<E, BM, BN, WM, WN, MI>;
+  using AccV = typename WT::AccV;
   constexpr int VEC = Tr::VEC;
   constexpr int NG = 64 / MI;       // lane groups along k
   constexpr int CK = NG * VEC;      // k per fragment chunk
-  constexpr int NVAL = MI * MI / 64;  // accumulator values per lane per block
   static_assert(BK % CK == 0, "BK must be a multiple of the chunk");
-  static_assert(WM * WN == 4, "four waves");
-  constexpr int WTM = BM / WM, WTN = BN / WN;
-  constexpr int FM = WTM / MI, FN = WTN / MI;
-  static_assert(FM * MI == WTM && FN * MI == WTN, "wave tile");
+  constexpr int NTH = 64 * WM * WN;  // workgroup size
+  constexpr int WTM = WT::WTM, WTN = WT::WTN, FM = WT::FM, FN = WT::FN;
 
-  using LA = TileLoader<T, BM, BK, A_K, VA>;
-  using LB = TileLoader<T, BN, BK, B_K, VB>;
+  using LA = TileLoader<T, BM, BK, A_K, VA, NTH>;
+  using LB = TileLoader<T, BN, BK, B_K, VB, NTH>;
   constexpr int ROWB = LA::ROWB;
   constexpr int A_BYTES = BM * ROWB, B_BYTES = BN * ROWB;
 
@@ -373,7 +383,6 @@ __device__ __forceinline__ void gemm_tile(
   const int wm = wave / WN, wn = wave % WN;
   const int lrow = lane % MI, lgrp = lane / MI;
 
-  AccV acc[FM][FN];
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -441,32 +450,63 @@ __device__ __forceinline__ void gemm_tile(
     }
     __syncthreads();
   }
+}
 
-  // Epilogue: C = alpha*acc + beta*C; beta == 0 never reads C.
-  const Acc alpha = (Acc)alpha_d, beta = (Acc)beta_d;
+// Calls f(row, col, value) for every accumulator value of this lane that
+// falls inside M x N (row / col are absolute output coordinates).
+template <Elem E, int BM, int BN, int WM, int WN, int MI, typename AccArr,
+          typename F>
+__device__ __forceinline__ void for_each_acc(const AccArr &acc, int M, int N,
+                                             int m0, int n0, F &&f) {
+  using WT = WaveTile<E, BM, BN, WM, WN, MI>;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wm = wave / WN, wn = wave % WN;
+  const int lrow = lane % MI, lgrp = lane / MI;
 #pragma unroll
-  for (int i = 0; i < FM; ++i) {
+  for (int i = 0; i < WT::FM; ++i) {
 #pragma unroll
-    for (int j = 0; j < FN; ++j) {
-      const int col = n0 + wn * WTN + j * MI + lrow;
+    for (int j = 0; j < WT::FN; ++j) {
+      const int col = n0 + wn * WT::WTN + j * MI + lrow;
       if (col >= N) continue;
-      const int rowb = m0 + wm * WTM + i * MI;
+      const int rowb = m0 + wm * WT::WTM + i * MI;
 #pragma unroll
-      for (int r = 0; r < NVAL; ++r) {
+      for (int r = 0; r < WT::NVAL; ++r) {
         const int row = rowb + out_row<E, MI>(lgrp, r);
         if (row >= M) continue;
-        OutT *dst = C + (int64_t)col * ldc + row;
-        Acc v = alpha * (Acc)acc[i][j][r];
-        if (beta != Acc(0)) v = fma_acc(beta, from_out<Acc>(*dst), v);
-        *dst = to_out<OutT>(v);
+        f(row, col, acc[i][j][r]);
       }
     }
   }
 }
 
+// One BM x BN output tile at (m0, n0) of C = alpha*A'*B' + beta*C, the whole
+// K range; beta == 0 never reads C.
 template <Elem E, typename OutT, bool A_K, bool B_K, int BM, int BN, int BK,
           int WM, int WN, bool VA, bool VB, int MI = 16>
-__global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmParams p) {
+__device__ __forceinline__ void gemm_tile(
+    const typename ElemT<E>::T *__restrict__ A, int64_t lda,
+    const typename ElemT<E>::T *__restrict__ B, int64_t ldb, OutT *C,
+    int64_t ldc, int M, int N, int K, double alpha_d, double beta_d, int m0,
+    int n0, char *lds) {
+  using Acc = typename ElemT<E>::Acc;
+  using WT = WaveTile<E, BM, BN, WM, WN, MI>;
+  typename WT::AccV acc[WT::FM][WT::FN];
+  gemm_mainloop<E, A_K, B_K, BM, BN, BK, WM, WN, VA, VB, MI>(
+      A, lda, B, ldb, M, N, K, m0, n0, lds, acc);
+  const Acc alpha = (Acc)alpha_d, beta = (Acc)beta_d;
+  for_each_acc<E, BM, BN, WM, WN, MI>(
+      acc, M, N, m0, n0, [&](int row, int col, auto a) {
+        OutT *dst = C + (int64_t)col * ldc + row;
+        Acc v = alpha * (Acc)a;
+        if (beta != Acc(0)) v = fma_acc(beta, from_out<Acc>(*dst), v);
+        *dst = to_out<OutT>(v);
+      });
+}
+
+template <Elem E, typename OutT, bool A_K, bool B_K, int BM, int BN, int BK,
+          int WM, int WN, bool VA, bool VB, int MI = 16>
+__global__ __launch_bounds__(64 * WM * WN) void gemm_mfma_kernel(
+    GemmParams p) {
   using T = typename ElemT<E>::T;
   __shared__ __attribute__((aligned(16))) char lds[tile_lds_bytes<E, BM, BN, BK>()];
   const int z = blockIdx.z;

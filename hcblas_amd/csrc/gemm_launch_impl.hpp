@@ -25,7 +25,8 @@ struct TileCfg<Elem::BF16> {
   static constexpr int BK_BIG = 64, BK_SMALL = 64;
 };
 
-template <Elem E, typename OutT, int BM, int BN, int BK, int WM, int WN>
+template <Elem E, typename OutT, int BM, int BN, int BK, int WM, int WN,
+          int MI = 16>
 hcblasStatus_t launch_tiles(hipStream_t stream, GemmParams p, bool a_k,
                             bool b_k, bool vload, int64_t batch) {
   p.tiles_m = (int)cdiv(p.M, BM);
@@ -46,8 +47,8 @@ hcblasStatus_t launch_tiles(hipStream_t stream, GemmParams p, bool a_k,
     dim3 grid(tiles, 1, (unsigned)nb);
 #define HCB_GEMM_CASE(AK, BKC, VL)                                        \
   if (a_k == AK && b_k == BKC && vload == VL)                             \
-    gemm_mfma_kernel<E, OutT, AK, BKC, BM, BN, BK, WM, WN, VL, VL>        \
-        <<<grid, 256, 0, stream>>>(q);
+    gemm_mfma_kernel<E, OutT, AK, BKC, BM, BN, BK, WM, WN, VL, VL, MI>    \
+        <<<grid, 64 * WM * WN, 0, stream>>>(q);
     HCB_GEMM_CASE(false, false, false)
     HCB_GEMM_CASE(false, true, false)
     HCB_GEMM_CASE(true, false, false)

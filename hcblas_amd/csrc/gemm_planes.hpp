@@ -46,7 +46,8 @@ struct PlaneParams {
 
 template <Elem E, typename OutT, int BM, int BN, int BK, int WM, int WN,
           int MI = 16>
-__global__ __launch_bounds__(256) void gemm_planes_kernel(PlaneParams p) {
+__global__ __launch_bounds__(64 * WM * WN) void gemm_planes_kernel(
+    PlaneParams p) {
   using Tr = ElemT<E>;
   using T = typename Tr::T;
   using AccV = typename AccOf<E, MI>::type;
@@ -59,8 +60,9 @@ __global__ __launch_bounds__(256) void gemm_planes_kernel(PlaneParams p) {
   static_assert(BK % CK == 0, "BK");
   constexpr int WTM = BM / WM, WTN = BN / WN;
   constexpr int FM = WTM / MI, FN = WTN / MI;
-  using LA = TileLoader<T, BM, BK, true, true>;
-  using LB = TileLoader<T, BN, BK, true, true>;
+  constexpr int NTH = 64 * WM * WN;
+  using LA = TileLoader<T, BM, BK, true, true, NTH>;
+  using LB = TileLoader<T, BN, BK, true, true, NTH>;
   constexpr int ROWB = LA::ROWB;
   constexpr int A_BYTES = BM * ROWB, B_BYTES = BN * ROWB;
   __shared__ __attribute__((aligned(16))) char lds[2 * (A_BYTES + B_BYTES)];

@@ -19,7 +19,17 @@
 // 2^(-7(d-2)); their INT32 sums are exact while s*K*64*64 < 2^31, and K is
 // processed in chunks when it is larger. Only d <= s+1 is kept.
 // FP64 is used only for the exponent scan, the slicing and the recombination.
+//
+// Ozaki scheme II (CRT): A' = rint(A * 2^(b - e_m)) and B' likewise are
+// integers with |A'|, |B'| <= 2^b. For pairwise coprime moduli m_1..m_n
+// (all <= 256, product P) the symmetric residues of A' and B' are INT8, one
+// INT8 GEMM per modulus gives A'B' mod m_t exactly (INT32 while
+// K * 128 * 128 < 2^31), and Garner's mixed-radix reconstruction with
+// symmetric digits recovers A'B' exactly whenever K * 2^(2b) < P/2, which
+// fixes b. n = 16 moduli give b = 55 bits per operand at K = 8192 with 16
+// INT8 products, against 36 for eight slices of the scheme above.
 #include <climits>
+#include <cmath>
 #include <vector>
 
 #include "gemm_common.hpp"
@@ -161,12 +171,13 @@ __global__ __launch_bounds__(256) void ozaki_slice_kernel(
   }
 }
 
-template <Elem E, typename OutT, int BM, int BN, int BK>
+template <Elem E, typename OutT, int BM, int BN, int BK, int WM = 2,
+          int WN = 2, int MI = 16>
 hcblasStatus_t launch_planes(hipStream_t stream, PlaneParams p) {
   p.tiles_m = (int)cdiv(p.M, BM);
   p.tiles_n = (int)cdiv(p.N, BN);
-  gemm_planes_kernel<E, OutT, BM, BN, BK, 2, 2>
-      <<<p.tiles_m * p.tiles_n, 256, 0, stream>>>(p);
+  gemm_planes_kernel<E, OutT, BM, BN, BK, WM, WN, MI>
+      <<<p.tiles_m * p.tiles_n, 64 * WM * WN, 0, stream>>>(p);
   return launch_status();
 }
 
@@ -208,6 +219,227 @@ hcblasStatus_t batch_pointers(hcblasHandle_t h, const Canon &c, void *C,
   (void)h;
   return HCBLAS_STATUS_SUCCESS;
 }
+
+// ---------------------------------------------------------------------------
+// CRT (Ozaki scheme II) kernels.
+// ---------------------------------------------------------------------------
+constexpr int kMaxModuli = 20;
+// Pairwise coprime, descending: 2^8, 3*5*17, 11*23, 251, 13*19, primes,
+// 7*31, primes.
+constexpr int kModuli[kMaxModuli] = {256, 255, 253, 251, 247, 241, 239,
+                                     233, 229, 227, 223, 217, 211, 199,
+                                     197, 193, 191, 181, 179, 173};
+__constant__ int d_moduli[kMaxModuli] = {256, 255, 253, 251, 247, 241, 239,
+                                         233, 229, 227, 223, 217, 211, 199,
+                                         197, 193, 191, 181, 179, 173};
+
+// a^-1 mod m (gcd(a, m) = 1), extended Euclid.
+constexpr int inv_mod(int a, int m) {
+  int t = 0, nt = 1, r = m, nr = a % m;
+  while (nr != 0) {
+    const int q = r / nr;
+    const int t2 = t - q * nt;
+    t = nt;
+    nt = t2;
+    const int r2 = r - q * nr;
+    r = nr;
+    nr = r2;
+  }
+  return t < 0 ? t + m : t;
+}
+
+// inv[t][s] = m_s^-1 mod m_t for s < t (Garner's constants).
+struct CrtInverses {
+  int inv[kMaxModuli][kMaxModuli];
+  constexpr CrtInverses() : inv() {
+    for (int t = 0; t < kMaxModuli; ++t)
+      for (int s = 0; s < t; ++s)
+        inv[t][s] = inv_mod(kModuli[s] % kModuli[t], kModuli[t]);
+  }
+};
+
+template <int T>
+__device__ __forceinline__ constexpr int modulus() {
+  constexpr int m[kMaxModuli] = {256, 255, 253, 251, 247, 241, 239,
+                                 233, 229, 227, 223, 217, 211, 199,
+                                 197, 193, 191, 181, 179, 173};
+  return m[T];
+}
+template <int T, int S>
+__device__ __forceinline__ constexpr int garner_inv() {
+  constexpr CrtInverses tab{};
+  return tab.inv[T][S];
+}
+
+// Symmetric residue of v modulo M: [-M/2, (M-1)/2].
+template <int M>
+__device__ __forceinline__ int sym_mod(long long v) {
+  int q = (int)(v % M);
+  if (q > (M - 1) / 2)
+    q -= M;
+  else if (q < -(M / 2))
+    q += M;
+  return q;
+}
+
+template <int NMOD, int T = 0>
+__device__ __forceinline__ void write_residues(long long v, int8_t *row,
+                                               int64_t Kp) {
+  if constexpr (T < NMOD) {
+    row[(int64_t)T * Kp] = (int8_t)sym_mod<modulus<T>()>(v);
+    write_residues<NMOD, T + 1>(v, row, Kp);
+  }
+}
+
+// Residue planes P[r][t][0..Kp) of rint(X(r, k) * 2^(bits - e_r)).
+template <int NMOD>
+__global__ __launch_bounds__(256) void crt_residue_kernel(
+    int R, int K, const double *X, int64_t ld, bool kcont, const int *exps,
+    int bits, int8_t *P, int64_t Kp) {
+  __shared__ TileStage<double> st;
+  const int r0 = blockIdx.y * 32, k0 = blockIdx.x * 32;
+  st.load(X, ld, kcont, R, K, r0, k0);
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int i = ty; i < 32; i += 8) {
+    const int r = r0 + i, k = k0 + tx;
+    if (r >= R || k >= Kp) continue;
+    const int e = exps[r];
+    const double v =
+        e == INT_MIN ? 0.0 : __builtin_rint(__builtin_ldexp(st.v[i][tx], bits - e));
+    write_residues<NMOD>((long long)v, P + (int64_t)r * NMOD * Kp + k, Kp);
+  }
+}
+
+struct CrtParams {
+  int M, N;
+  int64_t Kp;
+  int nmod;
+  const int8_t *Ap, *Bp;  // residue planes (rows of nmod * Kp)
+  uint8_t *R;             // R[t][n][m]: (A'B')(m, n) mod m_t in [0, m_t)
+  int tiles_r, tiles_c;
+};
+
+// One INT8 GEMM per modulus (blockIdx.z = t). The tile is computed as
+// (A'B')^T so that the MFMA output lanes run along m, the contiguous
+// dimension of R.
+template <int BM, int BN, int BK, int WM, int WN, int MI>
+__global__ __launch_bounds__(64 * WM * WN) void crt_gemm_kernel(CrtParams p) {
+  __shared__ __attribute__((aligned(16)))
+  char lds[tile_lds_bytes<Elem::I8, BM, BN, BK>()];
+  using WT = WaveTile<Elem::I8, BM, BN, WM, WN, MI>;
+  const int t = blockIdx.z;
+  const int mod = d_moduli[t];
+  int tr, tc;
+  tile_coords(blockIdx.x, p.tiles_r, p.tiles_c, tr, tc);
+  const int64_t ld = (int64_t)p.nmod * p.Kp;
+  typename WT::AccV acc[WT::FM][WT::FN];
+  gemm_mainloop<Elem::I8, true, true, BM, BN, BK, WM, WN, true, true, MI>(
+      p.Bp + t * p.Kp, ld, p.Ap + t * p.Kp, ld, p.N, p.M, (int)p.Kp, tr * BM,
+      tc * BN, lds, acc);
+  uint8_t *R = p.R + (int64_t)t * p.M * p.N;
+  for_each_acc<Elem::I8, BM, BN, WM, WN, MI>(
+      acc, p.N, p.M, tr * BM, tc * BN, [&](int n, int m, int a) {
+        int u = a % mod;
+        if (u < 0) u += mod;
+        R[(int64_t)n * p.M + m] = (uint8_t)u;
+      });
+}
+
+// Garner digit t (symmetric) from residue u_t and the digits before it.
+template <int T, int S = 0>
+__device__ __forceinline__ int garner_step(int x, const int *v) {
+  if constexpr (S < T) {
+    constexpr int M = modulus<T>();
+    x = ((x - v[S] + 2 * M) * garner_inv<T, S>()) % M;
+    return garner_step<T, S + 1>(x, v);
+  } else {
+    return x;
+  }
+}
+template <int NMOD, int T = 0>
+__device__ __forceinline__ void garner_digits(const int *u, int *v) {
+  if constexpr (T < NMOD) {
+    constexpr int M = modulus<T>();
+    const int x = garner_step<T>(u[T], v);
+    v[T] = x > (M - 1) / 2 ? x - M : x;
+    garner_digits<NMOD, T + 1>(u, v);
+  }
+}
+template <int NMOD, int T>
+__device__ __forceinline__ double horner(const int *v) {
+  if constexpr (T == NMOD - 1)
+    return (double)v[T];
+  else
+    return __builtin_fma(horner<NMOD, T + 1>(v), (double)modulus<T>(),
+                         (double)v[T]);
+}
+
+// C(m, n) = alpha * 2^(e_m + f_n - shift) * CRT(R[.][n][m]) + beta * C(m, n).
+template <int NMOD>
+__global__ __launch_bounds__(256) void crt_combine_kernel(
+    int M, int N, const uint8_t *R, const int *ea, const int *eb, int shift,
+    double alpha, double beta, double *C, int64_t ldc) {
+  const int64_t MN = (int64_t)M * N;
+  const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= MN) return;
+  const int m = (int)(idx % M), n = (int)(idx / M);
+  int u[NMOD], v[NMOD];
+#pragma unroll
+  for (int t = 0; t < NMOD; ++t) u[t] = R[t * MN + idx];
+  garner_digits<NMOD>(u, v);
+  const double x = horner<NMOD, 0>(v);
+  const int re = ea[m], ce = eb[n];
+  const double val = (re == INT_MIN || ce == INT_MIN)
+                         ? 0.0
+                         : __builtin_ldexp(x, re + ce - shift);
+  double *dst = C + (int64_t)n * ldc + m;
+  double o = alpha * val;
+  if (beta != 0.0) o = __builtin_fma(beta, *dst, o);
+  *dst = o;
+}
+
+template <int NMOD>
+void launch_crt_residues(hipStream_t s, int R, int K, const double *X,
+                         int64_t ld, bool kcont, const int *exps, int bits,
+                         int8_t *P, int64_t Kp) {
+  dim3 g((unsigned)cdiv(Kp, 32), (unsigned)cdiv(R, 32));
+  crt_residue_kernel<NMOD><<<g, 256, 0, s>>>(R, K, X, ld, kcont, exps, bits,
+                                            P, Kp);
+}
+template <int NMOD>
+void launch_crt_combine(hipStream_t s, int M, int N, const uint8_t *R,
+                        const int *ea, const int *eb, int shift, double alpha,
+                        double beta, double *C, int64_t ldc) {
+  crt_combine_kernel<NMOD>
+      <<<(unsigned)cdiv((int64_t)M * N, 256), 256, 0, s>>>(
+          M, N, R, ea, eb, shift, alpha, beta, C, ldc);
+}
+
+// Runtime modulus count -> template instance (8..20).
+template <template <int> class F, typename... Args>
+bool with_nmod(int n, Args &&...args) {
+  switch (n) {
+#define HCB_NMOD(k) \
+  case k:           \
+    F<k>::run(args...); \
+    return true;
+    HCB_NMOD(8) HCB_NMOD(9) HCB_NMOD(10) HCB_NMOD(11) HCB_NMOD(12)
+    HCB_NMOD(13) HCB_NMOD(14) HCB_NMOD(15) HCB_NMOD(16) HCB_NMOD(17)
+    HCB_NMOD(18) HCB_NMOD(19) HCB_NMOD(20)
+#undef HCB_NMOD
+  }
+  return false;
+}
+template <int NMOD>
+struct ResidueLaunch {
+  template <typename... A>
+  static void run(A... a) { launch_crt_residues<NMOD>(a...); }
+};
+template <int NMOD>
+struct CombineLaunch {
+  template <typename... A>
+  static void run(A... a) { launch_crt_combine<NMOD>(a...); }
+};
 
 }  // namespace
 
@@ -353,6 +585,108 @@ hcblasStatus_t gemm_ozaki_int8(hcblasHandle_t h, const Canon &c, double alpha,
       p.alpha = alpha;
       p.beta = k0 == 0 ? beta : 1.0;
       st = launch_planes<Elem::I8, double, 128, 128, BK>(h->stream, p);
+      if (st != HCBLAS_STATUS_SUCCESS) return st;
+    }
+  }
+  return HCBLAS_STATUS_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
+// Bits per operand for n moduli and a K chunk of kc: the largest b with
+// kc * 2^(2b) < P / 2 (P = product of the moduli), at most 62.
+static int crt_bits(int nmod, int64_t kc) {
+  double log2p = 0.0;
+  for (int t = 0; t < nmod; ++t) log2p += std::log2((double)kModuli[t]);
+  const double room = log2p - 1.0 - std::log2((double)kc) - 1e-6;
+  int b = (int)std::floor(room / 2.0);
+  return b > 62 ? 62 : b;
+}
+
+hcblasStatus_t gemm_ozaki_crt(hcblasHandle_t h, const Canon &c, double alpha,
+                              double beta, double *C, int64_t ldc,
+                              int64_t cOff, int64_t strideC,
+                              void *const *Cptrs, int64_t batch) {
+  if (c.M == 0 || c.N == 0 || batch == 0) return HCBLAS_STATUS_SUCCESS;
+  if (alpha == 0.0 || c.K == 0)
+    return gemm_real<Elem::F64, double>(h, c, alpha, beta, C, ldc, cOff,
+                                        strideC, Cptrs, batch);
+  const int nmod = h->ozaki_moduli;
+  constexpr int BM = 128, BN = 128, BK = 128, WM = 2, WN = 2, MI = 16;
+  // INT32 exactness of one modulus product: kc * 128 * 128 < 2^31.
+  constexpr int64_t kmax = 65536;
+  const int64_t kchunk = std::min<int64_t>(cdiv(c.K, BK) * BK, kmax);
+  const int64_t Kp = kchunk;
+  const size_t a_bytes = (size_t)nmod * c.M * Kp;
+  const size_t b_bytes = (size_t)nmod * c.N * Kp;
+  const size_t r_bytes = (size_t)nmod * c.M * c.N;
+  const size_t e_bytes = sizeof(int) * ((size_t)c.M + c.N);
+  auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
+  if (ensure_workspace(h->split_ws, up(a_bytes) + up(b_bytes) + up(r_bytes) +
+                                        e_bytes) != hipSuccess)
+    return HCBLAS_STATUS_ALLOC_FAILED;
+  char *ws = static_cast<char *>(h->split_ws.ptr);
+  int8_t *Ap = reinterpret_cast<int8_t *>(ws);
+  int8_t *Bp = reinterpret_cast<int8_t *>(ws + up(a_bytes));
+  uint8_t *R = reinterpret_cast<uint8_t *>(ws + up(a_bytes) + up(b_bytes));
+  int *exps = reinterpret_cast<int *>(ws + up(a_bytes) + up(b_bytes) +
+                                      up(r_bytes));
+  int *ea = exps, *eb = exps + c.M;
+  BatchView bv;
+  hcblasStatus_t st =
+      batch_pointers(h, c, C, strideC, Cptrs, batch, sizeof(double), bv);
+  if (st != HCBLAS_STATUS_SUCCESS) return st;
+  for (int64_t z = 0; z < batch; ++z) {
+    double *Cz = static_cast<double *>(bv.C[z]) + cOff;
+    for (int64_t k0 = 0; k0 < c.K; k0 += kchunk) {
+      const int kc = (int)std::min<int64_t>(kchunk, c.K - k0);
+      const int bits = crt_bits(nmod, kc);
+      if (bits < 1) return HCBLAS_STATUS_INVALID_VALUE;
+      const double *A = static_cast<const double *>(bv.A[z]) + c.aOff +
+                        (c.a_k ? k0 : k0 * c.lda);
+      const double *B = static_cast<const double *>(bv.B[z]) + c.bOff +
+                        (c.b_k ? k0 : k0 * c.ldb);
+      fill_int_kernel<<<(unsigned)cdiv(c.M + c.N, 256), 256, 0, h->stream>>>(
+          exps, c.M + c.N, INT_MIN);
+      if (c.a_k)
+        row_exp_kcont_kernel<<<dim3(c.M, (unsigned)cdiv(kc, 1024)), 256, 0,
+                               h->stream>>>(c.M, kc, A, c.lda, ea);
+      else
+        row_exp_rcont_kernel<<<dim3((unsigned)cdiv(c.M, 256),
+                                    (unsigned)cdiv(kc, 256)),
+                               256, 0, h->stream>>>(c.M, kc, A, c.lda, ea);
+      if (c.b_k)
+        row_exp_kcont_kernel<<<dim3(c.N, (unsigned)cdiv(kc, 1024)), 256, 0,
+                               h->stream>>>(c.N, kc, B, c.ldb, eb);
+      else
+        row_exp_rcont_kernel<<<dim3((unsigned)cdiv(c.N, 256),
+                                    (unsigned)cdiv(kc, 256)),
+                               256, 0, h->stream>>>(c.N, kc, B, c.ldb, eb);
+      if (!with_nmod<ResidueLaunch>(nmod, h->stream, c.M, kc, A, c.lda,
+                                    c.a_k, (const int *)ea, bits, Ap, Kp) ||
+          !with_nmod<ResidueLaunch>(nmod, h->stream, c.N, kc, B, c.ldb,
+                                    c.b_k, (const int *)eb, bits, Bp, Kp))
+        return HCBLAS_STATUS_INVALID_VALUE;
+      st = launch_status();
+      if (st != HCBLAS_STATUS_SUCCESS) return st;
+      CrtParams p{};
+      p.M = c.M;
+      p.N = c.N;
+      p.Kp = Kp;
+      p.nmod = nmod;
+      p.Ap = Ap;
+      p.Bp = Bp;
+      p.R = R;
+      p.tiles_r = (int)cdiv(c.N, BM);
+      p.tiles_c = (int)cdiv(c.M, BN);
+      crt_gemm_kernel<BM, BN, BK, WM, WN, MI>
+          <<<dim3((unsigned)(p.tiles_r * p.tiles_c), 1, (unsigned)nmod),
+             64 * WM * WN, 0, h->stream>>>(p);
+      st = launch_status();
+      if (st != HCBLAS_STATUS_SUCCESS) return st;
+      with_nmod<CombineLaunch>(nmod, h->stream, c.M, c.N, (const uint8_t *)R,
+                               (const int *)ea, (const int *)eb, 2 * bits,
+                               alpha, k0 == 0 ? beta : 1.0, Cz, ldc);
+      st = launch_status();
       if (st != HCBLAS_STATUS_SUCCESS) return st;
     }
   }

@@ -97,6 +97,7 @@ hcblasStatus_t hcblasDestroy(hcblasHandle_t *handle) {
   if (h->gemv_tickets.ptr) (void)hipFree(h->gemv_tickets.ptr);
   if (h->gemm_ws.ptr) (void)hipFree(h->gemm_ws.ptr);
   if (h->split_ws.ptr) (void)hipFree(h->split_ws.ptr);
+  if (h->adj_ws.ptr) (void)hipFree(h->adj_ws.ptr);
   delete h;
   *handle = nullptr;
   return HCBLAS_STATUS_SUCCESS;
@@ -117,7 +118,8 @@ hcblasStatus_t hcblasGetStream(hcblasHandle_t handle, hipStream_t *stream) {
 
 hcblasStatus_t hcblasSetMathMode(hcblasHandle_t handle, int mode) {
   if (!handle_ok(handle)) return HCBLAS_STATUS_NOT_INITIALIZED;
-  if (mode & ~(HCBLAS_SPLIT_BF16_MATH | HCBLAS_OZAKI_INT8_MATH))
+  if (mode & ~(HCBLAS_SPLIT_BF16_MATH | HCBLAS_OZAKI_INT8_MATH |
+               HCBLAS_OZAKI_CRT_MATH))
     return HCBLAS_STATUS_INVALID_VALUE;
   handle->math_mode = mode;
   return HCBLAS_STATUS_SUCCESS;
@@ -134,6 +136,13 @@ hcblasStatus_t hcblasSetOzakiSlices(hcblasHandle_t handle, int slices) {
   if (!handle_ok(handle)) return HCBLAS_STATUS_NOT_INITIALIZED;
   if (slices < 3 || slices > 12) return HCBLAS_STATUS_INVALID_VALUE;
   handle->ozaki_slices = slices;
+  return HCBLAS_STATUS_SUCCESS;
+}
+
+hcblasStatus_t hcblasSetOzakiModuli(hcblasHandle_t handle, int moduli) {
+  if (!handle_ok(handle)) return HCBLAS_STATUS_NOT_INITIALIZED;
+  if (moduli < 8 || moduli > 20) return HCBLAS_STATUS_INVALID_VALUE;
+  handle->ozaki_moduli = moduli;
   return HCBLAS_STATUS_SUCCESS;
 }
 

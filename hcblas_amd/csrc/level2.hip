@@ -552,6 +552,31 @@ hcblasStatus_t ger_entry(hcblasHandle_t h, hcblasOrder_t order, int M, int N,
   return HCBLAS_STATUS_SUCCESS;
 }
 
+// Adjoint of A_out = alpha * x * y^T + A_in (no reference counterpart): for
+// a cotangent dA (M x N in the given order) the GEMV pair
+//   dx = alpha * dA * y,   dy = alpha * dA^T * x
+// (dA_in = dA is the identity and needs no work). dx / dy are written with
+// unit stride; either may be NULL.
+template <typename T>
+hcblasStatus_t ger_backward_entry(hcblasHandle_t h, hcblasOrder_t order,
+                                         int M, int N, const T *alpha,
+                                         const T *x, int64_t incx, const T *y,
+                                         int64_t incy, const T *dA,
+                                         int64_t ldda, T *dx, T *dy) {
+  if (!handle_ok(h)) return HCBLAS_STATUS_NOT_INITIALIZED;
+  const T zero = T(0);
+  hcblasStatus_t st = HCBLAS_STATUS_SUCCESS;
+  if (dx) {
+    st = gemv_entry<T>(h, order, HCBLAS_OP_N, M, N, alpha, dA, 0, ldda, y, 0,
+                       incy, &zero, dx, 0, 1, 1, false);
+    if (st != HCBLAS_STATUS_SUCCESS) return st;
+  }
+  if (dy)
+    st = gemv_entry<T>(h, order, HCBLAS_OP_T, M, N, alpha, dA, 0, ldda, x, 0,
+                       incx, &zero, dy, 0, 1, 1, false);
+  return st;
+}
+
 }  // namespace
 }  // namespace hcb
 
@@ -651,6 +676,25 @@ hcblasStatus_t hcblas_dger(hcblasHandle_t handle, hcblasOrder_t order, int M,
                            int64_t aOffset, int64_t lda) {
   return ger_entry<double>(handle, order, M, N, alpha, X, xOffset, incX, Y,
                            yOffset, incY, A, aOffset, lda, 1);
+}
+
+hcblasStatus_t hcblas_sger_backward(hcblasHandle_t handle,
+                                    hcblasOrder_t order, int M, int N,
+                                    const float *alpha, const float *X,
+                                    int64_t incX, const float *Y, int64_t incY,
+                                    const float *dA, int64_t ldda, float *dX,
+                                    float *dY) {
+  return ger_backward_entry<float>(handle, order, M, N, alpha, X, incX, Y,
+                                   incY, dA, ldda, dX, dY);
+}
+hcblasStatus_t hcblas_dger_backward(hcblasHandle_t handle,
+                                    hcblasOrder_t order, int M, int N,
+                                    const double *alpha, const double *X,
+                                    int64_t incX, const double *Y,
+                                    int64_t incY, const double *dA,
+                                    int64_t ldda, double *dX, double *dY) {
+  return ger_backward_entry<double>(handle, order, M, N, alpha, X, incX, Y,
+                                    incY, dA, ldda, dX, dY);
 }
 
 }  // extern "C"

@@ -96,12 +96,18 @@ typedef uint16_t hcblasHalf;
  *   DEFAULT          : plain FP32 / FP64 / FP16->FP32 MFMA kernels.
  *   SPLIT_BF16       : S/C GEMM run as three-way BF16 split products on BF16
  *                      MFMA with FP32 accumulation.
- *   OZAKI_INT8       : D/Z GEMM run as Ozaki exact INT8 slices on INT8 MFMA.
+ *   OZAKI_INT8       : D/Z GEMM run as Ozaki exact INT8 slices on INT8 MFMA
+ *                      (scheme I: s slices, s(s+1)/2 INT8 products).
+ *   OZAKI_CRT        : D/Z GEMM run as Ozaki scheme II: one INT8 GEMM per
+ *                      modulus of a set of pairwise coprime moduli <= 256,
+ *                      exact Chinese-remainder reconstruction in the
+ *                      epilogue. Takes precedence over OZAKI_INT8.
  *   Modes combine with bitwise OR. */
 typedef enum hcblasMathMode_t {
   HCBLAS_DEFAULT_MATH = 0,
   HCBLAS_SPLIT_BF16_MATH = 1,
-  HCBLAS_OZAKI_INT8_MATH = 2
+  HCBLAS_OZAKI_INT8_MATH = 2,
+  HCBLAS_OZAKI_CRT_MATH = 4
 } hcblasMathMode_t;
 
 /* ------------------------------------------------------------------------ */
@@ -133,6 +139,10 @@ hcblasStatus_t hcblasGetMathMode(hcblasHandle_t handle, int *mode);
 /* Number of INT8 slices per operand for the Ozaki path, 3..12
  * (no reference counterpart). Default 8. */
 hcblasStatus_t hcblasSetOzakiSlices(hcblasHandle_t handle, int slices);
+
+/* Number of moduli for the OZAKI_CRT path, 8..20 (no reference counterpart).
+ * Default 16: 55-bit operands at K = 8192, 16 INT8 products. */
+hcblasStatus_t hcblasSetOzakiModuli(hcblasHandle_t handle, int moduli);
 
 /* ------------------------------------------------------------------------ */
 /* Host <-> device helpers (hcblas.h:174-233, hcblas.cpp:142-312)            */
@@ -482,6 +492,60 @@ hcblasStatus_t hcblas_dgemm_ozaki_int8(hcblasHandle_t handle,
                                        const double *beta, double *C,
                                        int64_t ldc, int64_t aOffset,
                                        int64_t bOffset, int64_t cOffset);
+hcblasStatus_t hcblas_dgemm_ozaki_crt(hcblasHandle_t handle,
+                                      hcblasOrder_t order,
+                                      hcblasOperation_t typeA,
+                                      hcblasOperation_t typeB, int M, int N,
+                                      int K, const double *alpha,
+                                      const double *A, int64_t lda,
+                                      const double *B, int64_t ldb,
+                                      const double *beta, double *C,
+                                      int64_t ldc, int64_t aOffset,
+                                      int64_t bOffset, int64_t cOffset);
+
+/* ------------------------------------------------------------------------ */
+/* Adjoints (no reference counterpart)                                       */
+/* ------------------------------------------------------------------------ */
+
+/* Reverse-mode adjoints of hcblas_gemm_strided_batched,
+ *   C = alpha * op(A) * op(B) + beta * C_in,
+ * for a cotangent dC of C (same order; A, B, dC at offset 0):
+ *   dA    = alpha * dC * op(B)^T, stored like A (alpha * op(B) * dC^T when
+ *           op(A) = A^T);
+ *   dB    = alpha * op(A)^T * dC, stored like B (alpha * dC^T * op(A) when
+ *           op(B) = B^T);
+ *   dC_in = beta * dC.
+ * dtype HCBLAS_R_32F / HCBLAS_R_64F: every matrix of that type, alpha / beta
+ * point to float / double. HCBLAS_R_16BF: A and B are bfloat16, dC, dA, dB
+ * and dC_in are float (the forward's accumulation type), alpha / beta are
+ * float. dA, dB or dCin may be NULL to skip that adjoint. op = C is treated
+ * as T for these real types. The handle's math mode applies to the adjoint
+ * GEMMs as to any GEMM. */
+hcblasStatus_t hcblas_gemm_strided_batched_backward(
+    hcblasHandle_t handle, hcblasDatatype_t dtype, hcblasOrder_t order,
+    hcblasOperation_t typeA, hcblasOperation_t typeB, int M, int N, int K,
+    const void *alpha, const void *beta, const void *A, int64_t lda,
+    int64_t strideA, const void *B, int64_t ldb, int64_t strideB,
+    const void *dC, int64_t lddc, int64_t strideDC, void *dA, int64_t ldda,
+    int64_t strideDA, void *dB, int64_t lddb, int64_t strideDB, void *dCin,
+    int64_t lddcin, int64_t strideDCin, int batchCount);
+
+/* Adjoint of hcblas_sger / hcblas_dger, A_out = alpha * x * y^T + A_in, for a
+ * cotangent dA (M x N in the given order): dx = alpha * dA * y and
+ * dy = alpha * dA^T * x, written with unit stride (either may be NULL);
+ * dA_in = dA. */
+hcblasStatus_t hcblas_sger_backward(hcblasHandle_t handle,
+                                    hcblasOrder_t order, int M, int N,
+                                    const float *alpha, const float *X,
+                                    int64_t incX, const float *Y, int64_t incY,
+                                    const float *dA, int64_t ldda, float *dX,
+                                    float *dY);
+hcblasStatus_t hcblas_dger_backward(hcblasHandle_t handle,
+                                    hcblasOrder_t order, int M, int N,
+                                    const double *alpha, const double *X,
+                                    int64_t incX, const double *Y,
+                                    int64_t incY, const double *dA,
+                                    int64_t ldda, double *dX, double *dY);
 
 /* ------------------------------------------------------------------------ */
 /* Grouped GEMM (no reference counterpart; the reference's batched GEMM     */

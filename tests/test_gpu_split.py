@@ -1,5 +1,6 @@
 """Split-precision GEMM on the GPU (no reference counterpart):
-BF16 three-way split SGEMM / CGEMM and Ozaki INT8 DGEMM / ZGEMM.
+BF16 three-way split SGEMM / CGEMM, Ozaki INT8 slice DGEMM / ZGEMM ("d") and
+Ozaki scheme II (CRT over INT8 moduli) DGEMM / ZGEMM ("dc").
 
 Checks:
   - bit-exact on the reference's integer recipes (test/unit/hcblas_sgemm.cpp
@@ -28,7 +29,8 @@ def _split(hc, handle, p, o, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C,
            ldc):
     api = hc.api
     dA, dB, dC = to_dev(A), to_dev(B), to_dev(C)
-    fn = api.hcblas_sgemm_split_bf16 if p == "s" else api.hcblas_dgemm_ozaki_int8
+    fn = {"s": api.hcblas_sgemm_split_bf16, "d": api.hcblas_dgemm_ozaki_int8,
+          "dc": api.hcblas_dgemm_ozaki_crt}[p]
     st = fn(handle, o, ta, tb, M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc,
             0, 0, 0)
     assert st == 0
@@ -39,25 +41,26 @@ def _plain(hc, handle, p, o, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C,
            ldc):
     api = hc.api
     dA, dB, dC = to_dev(A), to_dev(B), to_dev(C)
-    st = getattr(api, f"hcblas_{p}gemm")(handle, o, ta, tb, M, N, K, alpha, dA,
+    st = getattr(api, f"hcblas_{p[0]}gemm")(handle, o, ta, tb, M, N, K, alpha, dA,
                                          lda, dB, ldb, beta, dC, ldc, 0, 0, 0)
     assert st == 0
     return to_host(dC)
 
 
-@pytest.mark.parametrize("p", ["s", "d"])
+@pytest.mark.parametrize("p", ["s", "d", "dc"])
 def test_reference_recipe_bit_exact(hc, handle, golden, p):
     g = golden("gemm_ref_int.npz")
     M, N, K = (int(v) for v in g["dims"])
+    q = p[0]
     for i in range(8):
-        o, ta, tb, lda, ldb, ldc = (int(v) for v in g[f"{p}_case{i}"])
-        got = _split(hc, handle, p, o, ta, tb, M, N, K, 1.0, g[f"{p}_A"], lda,
-                     g[f"{p}_B"], ldb, 1.0, g[f"{p}_C"], ldc)
-        assert bitwise_equal(got, g[f"{p}_expect{i}"]), f"case {i}"
+        o, ta, tb, lda, ldb, ldc = (int(v) for v in g[f"{q}_case{i}"])
+        got = _split(hc, handle, p, o, ta, tb, M, N, K, 1.0, g[f"{q}_A"], lda,
+                     g[f"{q}_B"], ldb, 1.0, g[f"{q}_C"], ldc)
+        assert bitwise_equal(got, g[f"{q}_expect{i}"]), f"case {i}"
 
 
 def _gen(orc, p, n, seed, wide):
-    if p == "s":
+    if p[0] == "s":
         return orc.wide_f32(n, seed) if wide else orc.uniform_f32(n, seed)
     return orc.wide_f64(n, seed) if wide else orc.uniform_f64(n, seed)
 
@@ -70,7 +73,7 @@ def _rel_errors(orc, o, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
     return np.abs(got[idx] - ref) / np.maximum(np.abs(ref), 1e-300)
 
 
-@pytest.mark.parametrize("p", ["s", "d"])
+@pytest.mark.parametrize("p", ["s", "d", "dc"])
 @pytest.mark.parametrize("M,N,K", [(33, 17, 45), (128, 256, 512), (300, 129, 1000)])
 def test_all_cases_within_plain_bound(hc, handle, orc, p, M, N, K):
     eps = np.finfo(np.float32 if p == "s" else np.float64).eps
@@ -97,7 +100,7 @@ def test_all_cases_within_plain_bound(hc, handle, orc, p, M, N, K):
                 assert np.max(np.abs(got[idx] - ref)) <= tol, (o, ta, tb)
 
 
-@pytest.mark.parametrize("p", ["s", "d"])
+@pytest.mark.parametrize("p", ["s", "d", "dc"])
 @pytest.mark.parametrize("wide", [False, True])
 @pytest.mark.parametrize("ta", [N_, T_])
 def test_error_no_worse_than_plain(hc, handle, orc, p, wide, ta):
@@ -130,7 +133,7 @@ def test_math_mode_routes_plain_entries(hc, handle, orc):
     lib = hc.load()
     api = hc.api
     M, N, K = 200, 150, 300
-    for p, mode in (("s", 1), ("d", 2)):
+    for p, mode in (("s", 1), ("d", 2), ("dc", 4), ("dc", 6)):
         A = _gen(orc, p, M * K, 5, False)
         B = _gen(orc, p, K * N, 6, False)
         C = _gen(orc, p, M * N, 7, False)
@@ -138,7 +141,7 @@ def test_math_mode_routes_plain_entries(hc, handle, orc):
                      B[:N * K], N, 0.5, C, M)
         assert lib.hcblasSetMathMode(handle.value, mode) == 0
         dC = to_dev(C)
-        st = getattr(api, f"hcblas{p.upper()}gemm")(handle, N_, T_, M, N, K,
+        st = getattr(api, f"hcblas{p[0].upper()}gemm")(handle, N_, T_, M, N, K,
                                                     1.25, to_dev(A), M,
                                                     to_dev(B[:N * K]), N, 0.5,
                                                     dC, M)
@@ -147,8 +150,8 @@ def test_math_mode_routes_plain_entries(hc, handle, orc):
         assert lib.hcblasSetMathMode(handle.value, 0) == 0
 
 
-@pytest.mark.parametrize("p", ["c", "z"])
-def test_complex_split_modes(hc, handle, orc, p):
+@pytest.mark.parametrize("p,mode", [("c", 1), ("z", 2), ("z", 4)])
+def test_complex_split_modes(hc, handle, orc, p, mode):
     lib = hc.load()
     api = hc.api
     M, N, K = 96, 80, 200
@@ -161,7 +164,7 @@ def test_complex_split_modes(hc, handle, orc, p):
 
     A, B, C = cx(M * K, 1), cx(K * N, 2), cx(M * N, 3)
     alpha, beta = 0.5 - 1j, 0.25 + 0.5j
-    assert lib.hcblasSetMathMode(handle.value, 1 if p == "c" else 2) == 0
+    assert lib.hcblasSetMathMode(handle.value, mode) == 0
     for ta in (N_, C_):
         dC = to_dev(C)
         lda = M if ta == N_ else K
@@ -202,3 +205,62 @@ def test_ozaki_slices_control_accuracy(hc, handle, orc):
     print("ozaki max rel err by slices 4,6,8,10:", errs)
     assert errs[0] > errs[1] > errs[2]
     assert errs[3] <= errs[2]
+
+
+def test_crt_moduli_control_accuracy(hc, handle, orc):
+    """More moduli -> more operand bits (24, 40, 55 and 62 at K = 2048 for
+    8, 12, 16, 20 moduli) -> smaller error; 16 moduli reach the plain FP64
+    kernel's error level."""
+    lib = hc.load()
+    M = N = 128
+    K = 2048
+    A = orc.uniform_f64(M * K, 1)
+    B = orc.uniform_f64(K * N, 2)
+    C = np.zeros(M * N)
+    rng = np.random.default_rng(3)
+    rows = rng.integers(0, M, 512)
+    cols = rng.integers(0, N, 512)
+    errs = []
+    for n in (8, 12, 16, 20):
+        assert lib.hcblasSetOzakiModuli(handle.value, n) == 0
+        got = _split(hc, handle, "dc", COL, N_, N_, M, N, K, 1.0, A, M, B, K,
+                     0.0, C, M)
+        errs.append(_rel_errors(orc, COL, N_, N_, M, N, K, 1.0, A, M, B, K,
+                                0.0, C, M, got, rows, cols).max())
+    assert lib.hcblasSetOzakiModuli(handle.value, 16) == 0
+    plain = _plain(hc, handle, "d", COL, N_, N_, M, N, K, 1.0, A, M, B, K, 0.0,
+                   C, M)
+    ep = _rel_errors(orc, COL, N_, N_, M, N, K, 1.0, A, M, B, K, 0.0, C, M,
+                     plain, rows, cols).max()
+    print("crt max rel err by moduli 8,12,16,20:", errs, "plain", ep)
+    assert errs[0] > errs[1] > errs[2]
+    assert errs[3] <= errs[2]
+    assert errs[2] <= ep
+    assert lib.hcblasSetOzakiModuli(handle.value, 7) == 3
+    assert lib.hcblasSetOzakiModuli(handle.value, 21) == 3
+
+
+def test_crt_zero_rows_and_k_chunks(hc, handle, orc):
+    """All-zero rows / columns (no exponent) give exact zeros; K above the
+    65536 chunk runs in two chunks with beta applied once."""
+    M, N, K = 40, 24, 70000
+    A = orc.uniform_f64(M * K, 21)
+    B = orc.uniform_f64(K * N, 22)
+    A = A.reshape(K, M)
+    A[:, 5] = 0.0  # row 5 of A (column-major M x K)
+    A = A.reshape(-1)
+    B = B.reshape(N, K)
+    B[7, :] = 0.0  # column 7 of B
+    B = B.reshape(-1)
+    C = orc.uniform_f64(M * N, 23)
+    got = _split(hc, handle, "dc", COL, N_, N_, M, N, K, 1.0, A, M, B, K, 0.5,
+                 C, M)
+    rows = np.repeat(np.arange(M), N)
+    cols = np.tile(np.arange(N), M)
+    ref = orc.gemm_sample_exact(COL, N_, N_, M, N, K, 1.0, A, M, B, K, 0.5, C,
+                                M, rows, cols)
+    idx = rows + cols * M
+    assert np.max(np.abs(got[idx] - ref)) <= 4 * K * 2.0 ** -52
+    Cm = got.reshape(N, M)
+    assert np.array_equal(Cm[:, 5], 0.5 * C.reshape(N, M)[:, 5])
+    assert np.array_equal(Cm[7, :], 0.5 * C.reshape(N, M)[7, :])
