@@ -343,6 +343,62 @@ def grouped_section(hc, h, torch, count=4096, reps=10):
             "roofline_frac": round(bound_ms / ms_g, 4), "plan": info}
 
 
+def adjoint_section(torch, reps=5):
+    """Differentiation configuration through the libtorch layer: forward and
+    backward of C = A @ B over 256 problems of 256 x 1024 x 512 in bf16 with
+    fp32 accumulation (loss sum(C o W)), and of a 4096^2 SGER."""
+    from hcblas_amd import torch_ext
+    torch_ext.load()
+    g = torch.Generator(device="cuda").manual_seed(9)
+    batch, M, N, K = 256, 256, 1024, 512
+    A = (torch.rand((batch, M, K), generator=g, device="cuda") * 2 - 1).to(torch.bfloat16)
+    B = (torch.rand((batch, K, N), generator=g, device="cuda") * 2 - 1).to(torch.bfloat16)
+    W = torch.rand((batch, M, N), generator=g, device="cuda") * 2 - 1
+    A.requires_grad_()
+    B.requires_grad_()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def fwd():
+        return torch_ext.gemm_batched(A, B)
+
+    def fwd_bwd():
+        C = torch_ext.gemm_batched(A, B)
+        C.backward(W)
+
+    out = {}
+    for name, fn in (("forward", fwd), ("forward_backward", fwd_bwd)):
+        fn()
+        torch.cuda.synchronize()
+        s.record()
+        for _ in range(reps):
+            fn()
+        e.record()
+        torch.cuda.synchronize()
+        out[name + "_ms"] = round(s.elapsed_time(e) / reps, 4)
+    flop = 2.0 * batch * M * N * K
+    out["forward_tflops"] = round(flop / out["forward_ms"] / 1e9, 1)
+    out["backward_tflops"] = round(
+        2 * flop / (out["forward_backward_ms"] - out["forward_ms"]) / 1e9, 1)
+    n = 4096
+    x = torch.rand(n, generator=g, device="cuda").requires_grad_()
+    y = torch.rand(n, generator=g, device="cuda").requires_grad_()
+    A0 = torch.rand((n, n), generator=g, device="cuda").requires_grad_()
+    W2 = torch.rand((n, n), generator=g, device="cuda")
+
+    def ger_fb():
+        torch_ext.ger(x, y, A0, 1.5).backward(W2)
+
+    ger_fb()
+    torch.cuda.synchronize()
+    s.record()
+    for _ in range(reps):
+        ger_fb()
+    e.record()
+    torch.cuda.synchronize()
+    out["sger_forward_backward_ms"] = round(s.elapsed_time(e) / reps, 4)
+    return out
+
+
 def cpu_baseline(oracle):
     """The oracle's C restatement (single thread) on a bounded sample of the
     bandwidth set: gemv/ger at 8192^2, level 1 at n = 2^25 / 2^24."""
@@ -434,6 +490,9 @@ def main():
         del ops
         torch.cuda.empty_cache()
         result["gemm"] = gemm_section(hc, h, torch, oracle, args.gemm_n)
+    if not args.no_gemm:
+        result["grouped"] = grouped_section(hc, h, torch)
+        result["adjoint"] = adjoint_section(torch)
     if not args.no_cpu:
         import oracle
         result["cpu_baseline"] = cpu_baseline(oracle)

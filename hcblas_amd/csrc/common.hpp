@@ -40,7 +40,7 @@ struct hcblasContext {
   hipStream_t stream = nullptr;
   int math_mode = HCBLAS_DEFAULT_MATH;
   int ozaki_slices = 8;
-  int ozaki_moduli = 16;
+  int ozaki_moduli = 0;  // 0: chosen per K (see gemm_split.hip)
 
   // dot / asum: per-workgroup partials, a ticket counter that the last
   // workgroup resets to zero, the device result and a pinned host mirror.

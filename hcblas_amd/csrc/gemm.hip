@@ -325,8 +325,11 @@ hcblasStatus_t gemm_real(hcblasHandle_t h, const Canon &c, double alpha,
   // bases come from cudaMalloc-style allocations (256-byte aligned), which the
   // reference's callers use (test-blas.cpp:2000-2018).
   constexpr int V = vec_of<T>();
+  // An r-contiguous operand also needs its row count to be a whole number
+  // of vectors, so that a vector is wholly inside or outside the rows.
   bool vload = c.lda % V == 0 && c.ldb % V == 0 && c.aOff % V == 0 &&
-               c.bOff % V == 0;
+               c.bOff % V == 0 && (c.a_k || c.M % V == 0) &&
+               (c.b_k || c.N % V == 0);
   if (!c.Aptrs) vload = vload && ptr_aligned(c.A, c.aOff, sizeof(T)) &&
                         (batch == 1 || c.strideA % V == 0);
   if (!c.Bptrs) vload = vload && ptr_aligned(c.B, c.bOff, sizeof(T)) &&

@@ -313,6 +313,9 @@ hcblasStatus_t hcblasGroupedPlanCreate(hcblasHandle_t handle,
     };
     (void)es;
     if (q.lda % vec || q.ldb % vec || !al(q.A) || !al(q.B)) vload = false;
+    // r-contiguous operands: whole vectors of rows (see TileLoader)
+    if ((!(q.flags & 1) && q.M % vec) || (!(q.flags & 2) && q.N % vec))
+      vload = false;
   }
   p->vload = vload;
   if (hipMalloc(&p->d_probs, sizeof(ProbDev) * std::max(1, count)) !=

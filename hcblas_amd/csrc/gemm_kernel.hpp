@@ -97,6 +97,10 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef int intx16 __attribute__((ext_vector_type(16)));
 
 // Accumulator of one MI x MI MFMA block (MI = 16 or 32).
+// 16-byte fragment / staging register as a native vector (the HIP uint4
+// struct defeats scalar replacement of the staging arrays).
+typedef unsigned int vec4u __attribute__((ext_vector_type(4)));
+
 template <Elem E, int MI>
 struct AccOf;
 template <>
@@ -125,7 +129,7 @@ struct AccOf<Elem::I8, 32> { using type = intx16; };
 //   BF16  v_mfma_f32_16x16x32_bf16 / v_mfma_f32_32x32x16_bf16
 //   INT8  v_mfma_i32_16x16x64_i8 / v_mfma_i32_32x32x32_i8
 template <Elem E, int MI, int S>
-__device__ __forceinline__ void mfma_step(const uint4 &a, const uint4 &b,
+__device__ __forceinline__ void mfma_step(const vec4u &a, const vec4u &b,
                                           typename AccOf<E, MI>::type &c) {
   if constexpr (E == Elem::F32) {
     const uint32_t ua = S == 0 ? a.x : S == 1 ? a.y : S == 2 ? a.z : a.w;
@@ -139,9 +143,9 @@ __device__ __forceinline__ void mfma_step(const uint4 &a, const uint4 &b,
   } else if constexpr (E == Elem::F64) {
     static_assert(MI == 16, "FP64 MFMA is 16x16 only");
     const double av =
-        bitcast<double>(S == 0 ? make_uint2(a.x, a.y) : make_uint2(a.z, a.w));
+        bitcast<double>(S == 0 ? a.xy : a.zw);
     const double bv =
-        bitcast<double>(S == 0 ? make_uint2(b.x, b.y) : make_uint2(b.z, b.w));
+        bitcast<double>(S == 0 ? b.xy : b.zw);
     c = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, c, 0, 0, 0);
   } else if constexpr (E == Elem::F16) {
     if constexpr (MI == 16)
@@ -187,7 +191,7 @@ struct TileLoader {
   // conflicts for 32 r-vectors of one k).
   static constexpr bool BLOCKED = !KCONT && (ROWS / VEC) % 8 == 0 &&
                                   BK % 8 == 0 && NV % 64 == 0;
-  uint4 reg[PER_THREAD];
+  vec4u reg[PER_THREAD];
 
   // (r, k) of vector v of the tile (r, k are element indices).
   __device__ __forceinline__ static void vec_pos(int v, int &r, int &k) {
@@ -217,7 +221,7 @@ struct TileLoader {
       if (KCONT) {
         const T *src = P + (int64_t)gr * ld + gk;
         if (VLOAD && gr < R && gk + VEC <= K) {
-          reg[i] = *reinterpret_cast<const uint4 *>(src);
+          reg[i] = *reinterpret_cast<const vec4u *>(src);
         } else {
           T t[VEC];
 #pragma unroll
@@ -228,7 +232,7 @@ struct TileLoader {
       } else {
         const T *src = P + (int64_t)gk * ld + gr;
         if (VLOAD && gk < K && gr + VEC <= R) {
-          reg[i] = *reinterpret_cast<const uint4 *>(src);
+          reg[i] = *reinterpret_cast<const vec4u *>(src);
         } else {
           T t[VEC];
 #pragma unroll
@@ -240,6 +244,43 @@ struct TileLoader {
     }
   }
 
+  // A K step that lies entirely inside [0, K): no K checks, and rows past R
+  // load row r0 instead (always inside the operand). Those rows only feed
+  // output rows / columns >= M / N, which the epilogue never stores, so they
+  // need no zeroing, and the loaded registers are first consumed by the LDS
+  // store after the MFMAs: the global latency stays hidden behind them.
+  // Requires VLOAD and, for r-contiguous operands, R % VEC == 0 (a vector
+  // is then either wholly inside or wholly outside [0, R)).
+  __device__ __forceinline__ void load_full(const T *__restrict__ P,
+                                            int64_t ld, int r0, int k0,
+                                            int R) {
+    static_assert(VLOAD, "vector loads only");
+#pragma unroll
+    for (int i = 0; i < PER_THREAD; ++i) {
+      const int v = threadIdx.x + i * NTH;
+      if (NV % NTH != 0 && v >= NV) continue;
+      int r, k;
+      vec_pos(v, r, k);
+      const int gr = r0 + r, gk = k0 + k;
+      const int rr = gr < R ? gr : r0;
+      const T *src = KCONT ? P + (int64_t)rr * ld + gk : P + (int64_t)gk * ld + rr;
+      reg[i] = *reinterpret_cast<const vec4u *>(src);
+    }
+  }
+
+  // Full steps take the branch-free path; the K tail the checked one.
+  __device__ __forceinline__ void load_step(const T *__restrict__ P,
+                                            int64_t ld, int r0, int k0, int R,
+                                            int K, bool full) {
+    if constexpr (VLOAD) {
+      if (full) {
+        load_full(P, ld, r0, k0, R);
+        return;
+      }
+    }
+    load(P, ld, r0, k0, R, K);
+  }
+
   __device__ __forceinline__ void store(char *lds) const {
 #pragma unroll
     for (int i = 0; i < PER_THREAD; ++i) {
@@ -248,7 +289,7 @@ struct TileLoader {
       int r, k;
       vec_pos(v, r, k);
       if (KCONT) {
-        *reinterpret_cast<uint4 *>(lds + r * ROWB + k * (int)sizeof(T)) =
+        *reinterpret_cast<vec4u *>(lds + r * ROWB + k * (int)sizeof(T)) =
             reg[i];
       } else {
         T t[VEC];
@@ -355,7 +396,7 @@ struct WaveTile {
 // kernel and the residue kernel, so all of them produce bit-identical sums
 // for the same K step.
 template <Elem E, bool A_K, bool B_K, int BM, int BN, int BK, int WM, int WN,
-          bool VA, bool VB, int MI>
+          bool VA, bool VB, int MI, bool KFULL = false>
 __device__ __forceinline__ void gemm_mainloop(
     const typename ElemT<E>::T *__restrict__ A, int64_t lda,
     const typename ElemT<E>::T *__restrict__ B, int64_t ldb, int M, int N,
@@ -391,33 +432,24 @@ __device__ __forceinline__ void gemm_mainloop(
   LA la;
   LB lb;
   const int nk = (K + BK - 1) / BK;
-  la.load(A, lda, m0, 0, M, K);
-  lb.load(B, ldb, n0, 0, N, K);
-  la.store(lds);
-  lb.store(lds + A_BYTES);
-  __syncthreads();
+  // Steps with no K tail (KFULL: the caller guarantees K % BK == 0).
+  const int nfull = KFULL ? nk : K / BK;
 
-  for (int kt = 0; kt < nk; ++kt) {
-    char *cur = lds + (kt & 1) * (A_BYTES + B_BYTES);
-    char *nxt = lds + ((kt + 1) & 1) * (A_BYTES + B_BYTES);
-    const bool more = kt + 1 < nk;
-    if (more) {
-      la.load(A, lda, m0, (kt + 1) * BK, M, K);
-      lb.load(B, ldb, n0, (kt + 1) * BK, N, K);
-    }
+  // MFMAs over one LDS buffer.
+  auto compute = [&](const char *cur) {
     const char *As = cur;
     const char *Bs = cur + A_BYTES;
 #pragma unroll
     for (int c = 0; c < BK / CK; ++c) {
-      uint4 af[FM], bf[FN];
+      vec4u af[FM], bf[FN];
       const int kbyte = (c * CK + lgrp * VEC) * (int)sizeof(T);
 #pragma unroll
       for (int i = 0; i < FM; ++i)
-        af[i] = *reinterpret_cast<const uint4 *>(
+        af[i] = *reinterpret_cast<const vec4u *>(
             As + (wm * WTM + i * MI + lrow) * ROWB + kbyte);
 #pragma unroll
       for (int j = 0; j < FN; ++j)
-        bf[j] = *reinterpret_cast<const uint4 *>(
+        bf[j] = *reinterpret_cast<const vec4u *>(
             Bs + (wn * WTN + j * MI + lrow) * ROWB + kbyte);
 #pragma unroll
       for (int i = 0; i < FM; ++i)
@@ -444,9 +476,41 @@ __device__ __forceinline__ void gemm_mainloop(
             mfma_step<E, MI, 3>(af[i], bf[j], acc[i][j]);
       }
     }
+  };
+  auto buf = [&](int kt) { return lds + (kt & 1) * (A_BYTES + B_BYTES); };
+
+  if (nk == 0) return;
+  la.load_step(A, lda, m0, 0, M, K, nfull > 0);
+  lb.load_step(B, ldb, n0, 0, N, K, nfull > 0);
+  la.store(lds);
+  lb.store(lds + A_BYTES);
+  __syncthreads();
+
+  // Steady state: the step being fetched is a full one, so the loads are
+  // straight-line code and their registers are first read by the LDS
+  // stores after the MFMAs.
+  int kt = 0;
+  for (; kt + 1 < nfull; ++kt) {
+    la.load_step(A, lda, m0, (kt + 1) * BK, M, K, true);
+    lb.load_step(B, ldb, n0, (kt + 1) * BK, N, K, true);
+    // Keep the loads ahead of every MFMA of the step.
+    __builtin_amdgcn_sched_barrier(0);
+    compute(buf(kt));
+    la.store(buf(kt + 1));
+    lb.store(buf(kt + 1) + A_BYTES);
+    __syncthreads();
+  }
+  // The last full step and the K tail (at most two steps).
+  for (; kt < nk; ++kt) {
+    const bool more = kt + 1 < nk;
     if (more) {
-      la.store(nxt);
-      lb.store(nxt + A_BYTES);
+      la.load(A, lda, m0, (kt + 1) * BK, M, K);
+      lb.load(B, ldb, n0, (kt + 1) * BK, N, K);
+    }
+    compute(buf(kt));
+    if (more) {
+      la.store(buf(kt + 1));
+      lb.store(buf(kt + 1) + A_BYTES);
     }
     __syncthreads();
   }

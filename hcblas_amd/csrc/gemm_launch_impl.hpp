@@ -64,22 +64,39 @@ hcblasStatus_t launch_tiles(hipStream_t stream, GemmParams p, bool a_k,
   return HCBLAS_STATUS_SUCCESS;
 }
 
-// Macro-tile choice (measured at 8192^3 on MI355X): FP32 runs 256x128 tiles
-// (110 TF/s, against 105 for 128x128); FP64 is occupancy-bound and runs
-// 64x64 tiles (55.6 TF/s, against 44 for 128x128). Problems that cannot fill
-// the CUs with large tiles drop to 64x64.
+// Macro-tile choice (measured at 8192^3 on MI355X, the largest tile whose
+// grid still covers every CU is taken):
+//   FP32  256x256x32, 8 waves of 128x64 on 32x32 MFMA blocks (134 TF/s,
+//         85% of peak), then 256x128 8 waves (128), 128x128 4 waves, 64x64;
+//   FP64  128x64x16, 8 waves (60 TF/s, 77%), then 64x64 4 waves (59);
+//   F16 / BF16 256x256x64, 8 waves on 32x32 blocks, then 128x128 4 waves,
+//         then 64x64.
 template <Elem E, typename OutT>
 hcblasStatus_t launch_gemm(hipStream_t stream, int num_cus, GemmParams p,
                            bool a_k, bool b_k, bool vload, int64_t batch) {
+  auto tiles = [&](int bm, int bn) {
+    return cdiv(p.M, bm) * cdiv(p.N, bn) * batch;
+  };
   if constexpr (E == Elem::F32) {
-    const int64_t big = cdiv(p.M, 256) * cdiv(p.N, 128) * batch;
-    if (big >= num_cus)
-      return launch_tiles<E, OutT, 256, 128, 32, 2, 2>(stream, p, a_k, b_k,
+    if (tiles(256, 256) >= num_cus)
+      return launch_tiles<E, OutT, 256, 256, 32, 2, 4, 32>(stream, p, a_k, b_k,
+                                                           vload, batch);
+    if (tiles(256, 128) >= num_cus)
+      return launch_tiles<E, OutT, 256, 128, 32, 2, 4>(stream, p, a_k, b_k,
                                                        vload, batch);
-  } else if constexpr (E != Elem::F64) {
-    const int64_t big = cdiv(p.M, 128) * cdiv(p.N, 128) * batch;
-    if (big >= num_cus)
-      return launch_tiles<E, OutT, 128, 128, TileCfg<E>::BK_BIG, 2, 2>(
+    if (tiles(128, 128) >= num_cus)
+      return launch_tiles<E, OutT, 128, 128, 32, 2, 2>(stream, p, a_k, b_k,
+                                                       vload, batch);
+  } else if constexpr (E == Elem::F64) {
+    if (tiles(128, 64) >= num_cus)
+      return launch_tiles<E, OutT, 128, 64, 16, 2, 4>(stream, p, a_k, b_k,
+                                                      vload, batch);
+  } else {
+    if (tiles(256, 256) >= num_cus)
+      return launch_tiles<E, OutT, 256, 256, TileCfg<E>::BK_BIG, 2, 4, 32>(
+          stream, p, a_k, b_k, vload, batch);
+    if (tiles(128, 128) >= num_cus)
+      return launch_tiles<E, OutT, 128, 128, TileCfg<E>::BK_BIG, 2, 2, 32>(
           stream, p, a_k, b_k, vload, batch);
   }
   return launch_tiles<E, OutT, 64, 64, TileCfg<E>::BK_SMALL, 2, 2>(

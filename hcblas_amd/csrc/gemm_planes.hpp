@@ -104,8 +104,8 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_planes_kernel(
   auto load_tile = [&](int g, int kt) {
     const int64_t ka = (int64_t)p.a0[g] * p.Kp + (int64_t)kt * BK;
     const int64_t kb = (int64_t)p.b0[g] * p.Kp + (int64_t)kt * BK;
-    la.load(Ap + ka, ld, m0, 0, p.M, BK);
-    lb.load(Bp + kb, ld, n0, 0, p.N, BK);
+    la.load_full(Ap + ka, ld, m0, 0, p.M);
+    lb.load_full(Bp + kb, ld, n0, 0, p.N);
   };
   load_tile(0, 0);
   la.store(lds);
@@ -126,19 +126,20 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_planes_kernel(
       }
       load_tile(lg, lkt);
     }
+    __builtin_amdgcn_sched_barrier(0);
     const char *As = cur;
     const char *Bs = cur + A_BYTES;
 #pragma unroll
     for (int c = 0; c < BK / CK; ++c) {
-      uint4 af[FM], bf[FN];
+      vec4u af[FM], bf[FN];
       const int kbyte = (c * CK + lgrp * VEC) * (int)sizeof(T);
 #pragma unroll
       for (int i = 0; i < FM; ++i)
-        af[i] = *reinterpret_cast<const uint4 *>(
+        af[i] = *reinterpret_cast<const vec4u *>(
             As + (wm * WTM + i * MI + lrow) * ROWB + kbyte);
 #pragma unroll
       for (int j = 0; j < FN; ++j)
-        bf[j] = *reinterpret_cast<const uint4 *>(
+        bf[j] = *reinterpret_cast<const vec4u *>(
             Bs + (wn * WTN + j * MI + lrow) * ROWB + kbyte);
 #pragma unroll
       for (int i = 0; i < FM; ++i)

@@ -333,9 +333,9 @@ __global__ __launch_bounds__(64 * WM * WN) void crt_gemm_kernel(CrtParams p) {
   tile_coords(blockIdx.x, p.tiles_r, p.tiles_c, tr, tc);
   const int64_t ld = (int64_t)p.nmod * p.Kp;
   typename WT::AccV acc[WT::FM][WT::FN];
-  gemm_mainloop<Elem::I8, true, true, BM, BN, BK, WM, WN, true, true, MI>(
-      p.Bp + t * p.Kp, ld, p.Ap + t * p.Kp, ld, p.N, p.M, (int)p.Kp, tr * BM,
-      tc * BN, lds, acc);
+  gemm_mainloop<Elem::I8, true, true, BM, BN, BK, WM, WN, true, true, MI,
+                true>(p.Bp + t * p.Kp, ld, p.Ap + t * p.Kp, ld, p.N, p.M,
+                      (int)p.Kp, tr * BM, tc * BN, lds, acc);
   uint8_t *R = p.R + (int64_t)t * p.M * p.N;
   for_each_acc<Elem::I8, BM, BN, WM, WN, MI>(
       acc, p.N, p.M, tr * BM, tc * BN, [&](int n, int m, int a) {
@@ -365,13 +365,36 @@ __device__ __forceinline__ void garner_digits(const int *u, int *v) {
     garner_digits<NMOD, T + 1>(u, v);
   }
 }
+// Mixed-radix value sum_t v_t * prod_{s<t} m_s by Horner's rule in
+// double-double (about 106 bits; the exact value has up to ~126), so the
+// returned double is the correctly rounded value except within 2^-105 of a
+// rounding boundary, and exactly representable results come out exact.
 template <int NMOD, int T>
+__device__ __forceinline__ void horner_dd(const int *v, double &hi,
+                                          double &lo) {
+  if constexpr (T == NMOD - 1) {
+    hi = (double)v[T];
+    lo = 0.0;
+  } else {
+#pragma clang fp contract(off)
+    horner_dd<NMOD, T + 1>(v, hi, lo);
+    const double m = (double)modulus<T>();
+    const double p = hi * m;
+    const double e = __builtin_fma(hi, m, -p);  // hi*m = p + e exactly
+    const double vt = (double)v[T];
+    const double s = p + vt;                    // two-sum: p + vt = s + t
+    const double bb = s - p;
+    const double t = (p - (s - bb)) + (vt - bb);
+    const double l = __builtin_fma(lo, m, e + t);
+    hi = s + l;                                 // renormalise
+    lo = l - (hi - s);
+  }
+}
+template <int NMOD>
 __device__ __forceinline__ double horner(const int *v) {
-  if constexpr (T == NMOD - 1)
-    return (double)v[T];
-  else
-    return __builtin_fma(horner<NMOD, T + 1>(v), (double)modulus<T>(),
-                         (double)v[T]);
+  double hi, lo;
+  horner_dd<NMOD, 0>(v, hi, lo);
+  return hi + lo;
 }
 
 // C(m, n) = alpha * 2^(e_m + f_n - shift) * CRT(R[.][n][m]) + beta * C(m, n).
@@ -387,7 +410,7 @@ __global__ __launch_bounds__(256) void crt_combine_kernel(
 #pragma unroll
   for (int t = 0; t < NMOD; ++t) u[t] = R[t * MN + idx];
   garner_digits<NMOD>(u, v);
-  const double x = horner<NMOD, 0>(v);
+  const double x = horner<NMOD>(v);
   const int re = ea[m], ce = eb[n];
   const double val = (re == INT_MIN || ce == INT_MIN)
                          ? 0.0
@@ -492,7 +515,14 @@ hcblasStatus_t gemm_split_bf16(hcblasHandle_t h, const Canon &c, float alpha,
     p.ldc = ldc;
     p.alpha = alpha;
     p.beta = beta;
-    st = launch_planes<Elem::BF16, float, 128, 128, BK>(h->stream, p);
+    // 256x256 tiles, 8 waves on 32x32 BF16 MFMA blocks (42% of BF16 peak at
+    // 8192^3); smaller problems that cannot fill the CUs take 128x128.
+    if (cdiv(c.M, 256) * cdiv(c.N, 256) >= h->num_cus)
+      st = launch_planes<Elem::BF16, float, 256, 256, BK, 2, 4, 32>(h->stream,
+                                                                    p);
+    else
+      st = launch_planes<Elem::BF16, float, 128, 128, BK, 2, 2, 32>(h->stream,
+                                                                    p);
     if (st != HCBLAS_STATUS_SUCCESS) return st;
   }
   return HCBLAS_STATUS_SUCCESS;
@@ -602,6 +632,17 @@ static int crt_bits(int nmod, int64_t kc) {
   return b > 62 ? 62 : b;
 }
 
+// Default modulus count: the fewest moduli whose operand precision matches
+// the FP64 kernel's error at this K. A K-long FP64 dot product carries a
+// relative error of about sqrt(K) * 2^-53; operands rounded to b bits carry
+// about 2^-b, so b >= 54 - log2(K) / 2 (48 bits, 14 moduli at K = 8192).
+static int crt_auto_moduli(int64_t kc) {
+  const int need = (int)std::ceil(54.0 - 0.5 * std::log2((double)std::max<int64_t>(kc, 1)));
+  for (int n = 8; n <= kMaxModuli; ++n)
+    if (crt_bits(n, kc) >= need) return n;
+  return kMaxModuli;
+}
+
 hcblasStatus_t gemm_ozaki_crt(hcblasHandle_t h, const Canon &c, double alpha,
                               double beta, double *C, int64_t ldc,
                               int64_t cOff, int64_t strideC,
@@ -610,15 +651,21 @@ hcblasStatus_t gemm_ozaki_crt(hcblasHandle_t h, const Canon &c, double alpha,
   if (alpha == 0.0 || c.K == 0)
     return gemm_real<Elem::F64, double>(h, c, alpha, beta, C, ldc, cOff,
                                         strideC, Cptrs, batch);
-  const int nmod = h->ozaki_moduli;
-  constexpr int BM = 128, BN = 128, BK = 128, WM = 2, WN = 2, MI = 16;
+  // 256x256 tiles, 8 waves on 16x16 INT8 MFMA blocks (38.5% of INT8 peak
+  // at 8192^2 x 16 moduli, against 28.8% for 128x128).
+  constexpr int BM = 256, BN = 256, BK = 128, WM = 2, WN = 4, MI = 16;
   // INT32 exactness of one modulus product: kc * 128 * 128 < 2^31.
   constexpr int64_t kmax = 65536;
   const int64_t kchunk = std::min<int64_t>(cdiv(c.K, BK) * BK, kmax);
   const int64_t Kp = kchunk;
-  const size_t a_bytes = (size_t)nmod * c.M * Kp;
-  const size_t b_bytes = (size_t)nmod * c.N * Kp;
-  const size_t r_bytes = (size_t)nmod * c.M * c.N;
+  // Workspace for the largest modulus count any chunk uses.
+  const int nmax = h->ozaki_moduli
+                       ? h->ozaki_moduli
+                       : std::max(crt_auto_moduli((int)kchunk),
+                                  crt_auto_moduli((int)(c.K % kchunk ? c.K % kchunk : kchunk)));
+  const size_t a_bytes = (size_t)nmax * c.M * Kp;
+  const size_t b_bytes = (size_t)nmax * c.N * Kp;
+  const size_t r_bytes = (size_t)nmax * c.M * c.N;
   const size_t e_bytes = sizeof(int) * ((size_t)c.M + c.N);
   auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
   if (ensure_workspace(h->split_ws, up(a_bytes) + up(b_bytes) + up(r_bytes) +
@@ -639,6 +686,7 @@ hcblasStatus_t gemm_ozaki_crt(hcblasHandle_t h, const Canon &c, double alpha,
     double *Cz = static_cast<double *>(bv.C[z]) + cOff;
     for (int64_t k0 = 0; k0 < c.K; k0 += kchunk) {
       const int kc = (int)std::min<int64_t>(kchunk, c.K - k0);
+      const int nmod = h->ozaki_moduli ? h->ozaki_moduli : crt_auto_moduli(kc);
       const int bits = crt_bits(nmod, kc);
       if (bits < 1) return HCBLAS_STATUS_INVALID_VALUE;
       const double *A = static_cast<const double *>(bv.A[z]) + c.aOff +

@@ -140,8 +140,10 @@ hcblasStatus_t hcblasGetMathMode(hcblasHandle_t handle, int *mode);
  * (no reference counterpart). Default 8. */
 hcblasStatus_t hcblasSetOzakiSlices(hcblasHandle_t handle, int slices);
 
-/* Number of moduli for the OZAKI_CRT path, 8..20 (no reference counterpart).
- * Default 16: 55-bit operands at K = 8192, 16 INT8 products. */
+/* Number of moduli for the OZAKI_CRT path, 8..20, or 0 (the default) to
+ * pick, per K, the fewest moduli whose operand precision matches the plain
+ * FP64 kernel's error: b >= 54 - log2(K)/2 bits, i.e. 14 moduli (48 bits)
+ * at K = 8192 (no reference counterpart). */
 hcblasStatus_t hcblasSetOzakiModuli(hcblasHandle_t handle, int moduli);
 
 /* ------------------------------------------------------------------------ */
