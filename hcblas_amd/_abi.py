@@ -21,7 +21,9 @@ _ENUMS = {
     "hcblasOrder_t",
     "hcblasMathMode_t",
     "hcblasDatatype_t",
+    "hipblasOperation_t",
 }
+SHIM_HEADER = os.path.join(REPO_ROOT, "include", "hipblas_hcblas.h")
 
 
 @dataclass(frozen=True)
@@ -40,19 +42,22 @@ def _ctype(ctype: str):
         return ctypes.c_int
     if t == "int64_t":
         return ctypes.c_int64
-    if t in ("hcblasHandle_t", "hipStream_t", "hcblasGroupedPlan_t"):
+    if t in ("hcblasHandle_t", "hipStream_t", "hcblasGroupedPlan_t",
+             "hipblasHandle_t"):
         return ctypes.c_void_p
     raise ValueError(f"unmapped C type: {ctype!r}")
 
 
-def parse_header(path: str = HEADER) -> Dict[str, Prototype]:
-    """Returns every ``hcblasStatus_t name(args);`` declared in the header."""
+def parse_header(path: str = HEADER,
+                 ret: str = "hcblasStatus_t") -> Dict[str, Prototype]:
+    """Returns every ``<ret> name(args);`` declared in the header
+    (``hipblasStatus_t`` for the hipBLAS-compatible header)."""
     with open(path, "r", encoding="utf-8") as f:
         text = f.read()
     text = re.sub(r"/\*.*?\*/", " ", text, flags=re.S)
     text = re.sub(r"//[^\n]*", " ", text)
     protos: Dict[str, Prototype] = {}
-    for m in re.finditer(r"hcblasStatus_t\s+(\w+)\s*\(([^;{]*?)\)\s*;", text):
+    for m in re.finditer(ret + r"\s+(\w+)\s*\(([^;{]*?)\)\s*;", text):
         name, argtext = m.group(1), m.group(2)
         args: List[Tuple[str, str]] = []
         for raw in argtext.split(","):

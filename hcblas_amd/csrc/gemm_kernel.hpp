@@ -30,6 +30,10 @@
 //     a GROUP_M-grouped tile order (A and B panels stay in that XCD's L2).
 #pragma once
 
+#ifndef HCB_EARLY_STORE
+#define HCB_EARLY_STORE 0
+#endif
+
 #include "common.hpp"
 
 namespace hcb {
@@ -480,6 +484,38 @@ __device__ __forceinline__ void gemm_mainloop(
   auto buf = [&](int kt) { return lds + (kt & 1) * (A_BYTES + B_BYTES); };
 
   if (nk == 0) return;
+#if HCB_EARLY_STORE
+  // Pipeline: the registers hold step kt+1 while step kt computes. At the
+  // top of step kt they are written to the other LDS buffer (free: every
+  // wave passed the previous barrier) and refilled with step kt+2, so the
+  // LDS writes overlap this step's MFMAs and one barrier ends the step.
+  auto load = [&](int k, bool full) {
+    la.load_step(A, lda, m0, k * BK, M, K, full);
+    lb.load_step(B, ldb, n0, k * BK, N, K, full);
+  };
+  auto store = [&](int k) {
+    la.store(buf(k));
+    lb.store(buf(k) + A_BYTES);
+  };
+  load(0, nfull > 0);
+  store(0);
+  if (nk > 1) load(1, 1 < nfull);
+  __syncthreads();
+  int kt = 0;
+  for (; kt + 2 < nfull; ++kt) {
+    store(kt + 1);
+    load(kt + 2, true);
+    __builtin_amdgcn_sched_barrier(0);
+    compute(buf(kt));
+    __syncthreads();
+  }
+  for (; kt < nk; ++kt) {
+    if (kt + 1 < nk) store(kt + 1);
+    if (kt + 2 < nk) load(kt + 2, kt + 2 < nfull);
+    compute(buf(kt));
+    __syncthreads();
+  }
+#else
   la.load_step(A, lda, m0, 0, M, K, nfull > 0);
   lb.load_step(B, ldb, n0, 0, N, K, nfull > 0);
   la.store(lds);
@@ -514,6 +550,7 @@ __device__ __forceinline__ void gemm_mainloop(
     }
     __syncthreads();
   }
+#endif
 }
 
 // Calls f(row, col, value) for every accumulator value of this lane that

@@ -99,34 +99,25 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_planes_kernel(
 
   LA la;
   LB lb;
-  // (g, kt) cursors for the tile being loaded.
-  int lg = 0, lkt = 0;
   auto load_tile = [&](int g, int kt) {
     const int64_t ka = (int64_t)p.a0[g] * p.Kp + (int64_t)kt * BK;
     const int64_t kb = (int64_t)p.b0[g] * p.Kp + (int64_t)kt * BK;
     la.load_full(Ap + ka, ld, m0, 0, p.M);
     lb.load_full(Bp + kb, ld, n0, 0, p.N);
   };
-  load_tile(0, 0);
-  la.store(lds);
-  lb.store(lds + A_BYTES);
-  __syncthreads();
-  int cg = 0, ckt = 0;  // group / tile being computed
-  lg = 0;
-  lkt = 0;
-
-  for (int t = 0; t < total; ++t) {
-    char *cur = lds + (t & 1) * (A_BYTES + B_BYTES);
-    char *nxt = lds + ((t + 1) & 1) * (A_BYTES + B_BYTES);
-    const bool more = t + 1 < total;
-    if (more) {
-      if (++lkt == p.np[lg] * kt_per_plane) {
-        lkt = 0;
-        ++lg;
-      }
-      load_tile(lg, lkt);
+  auto buf = [&](int t) { return lds + (t & 1) * (A_BYTES + B_BYTES); };
+  auto store = [&](int t) {
+    la.store(buf(t));
+    lb.store(buf(t) + A_BYTES);
+  };
+  // Advances a (group, k-tile) cursor by one tile of the flat sequence.
+  auto advance = [&](int &g, int &kt) {
+    if (++kt == p.np[g] * kt_per_plane) {
+      kt = 0;
+      ++g;
     }
-    __builtin_amdgcn_sched_barrier(0);
+  };
+  auto compute = [&](const char *cur) {
     const char *As = cur;
     const char *Bs = cur + A_BYTES;
 #pragma unroll
@@ -147,11 +138,10 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_planes_kernel(
         for (int j = 0; j < FN; ++j)
           mfma_step<E, MI, 0>(af[i], bf[j], acc[i][j]);
     }
-    if (more) {
-      la.store(nxt);
-      lb.store(nxt + A_BYTES);
-    }
-    // End of a group: INT32 group sums are exact; fold them into FP64.
+  };
+  int cg = 0, ckt = 0;  // group / tile being computed
+  // End of a group: INT32 group sums are exact; fold them into FP64.
+  auto finish_tile = [&]() {
     if (++ckt == p.np[cg] * kt_per_plane) {
       if constexpr (INT) {
         const double w = p.weight[cg];
@@ -169,8 +159,48 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_planes_kernel(
       ckt = 0;
       ++cg;
     }
+  };
+
+#if HCB_EARLY_STORE
+  // Registers hold tile t+1 while tile t computes; they are written to the
+  // other LDS buffer at the top of the step and refilled with tile t+2.
+  int lg = 0, lkt = 0;  // cursor of the tile in the registers
+  load_tile(0, 0);
+  store(0);
+  if (total > 1) {
+    advance(lg, lkt);
+    load_tile(lg, lkt);
+  }
+  __syncthreads();
+  for (int t = 0; t < total; ++t) {
+    if (t + 1 < total) store(t + 1);
+    if (t + 2 < total) {
+      advance(lg, lkt);
+      load_tile(lg, lkt);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    compute(buf(t));
+    finish_tile();
     __syncthreads();
   }
+#else
+  int lg = 0, lkt = 0;  // cursor of the tile being loaded
+  load_tile(0, 0);
+  store(0);
+  __syncthreads();
+  for (int t = 0; t < total; ++t) {
+    const bool more = t + 1 < total;
+    if (more) {
+      advance(lg, lkt);
+      load_tile(lg, lkt);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    compute(buf(t));
+    if (more) store(t + 1);
+    finish_tile();
+    __syncthreads();
+  }
+#endif
 
   OutT *C = static_cast<OutT *>(p.C);
 #pragma unroll

@@ -337,10 +337,16 @@ __global__ __launch_bounds__(64 * WM * WN) void crt_gemm_kernel(CrtParams p) {
                 true>(p.Bp + t * p.Kp, ld, p.Ap + t * p.Kp, ld, p.N, p.M,
                       (int)p.Kp, tr * BM, tc * BN, lds, acc);
   uint8_t *R = p.R + (int64_t)t * p.M * p.N;
+  // a mod m without an integer division: |a| <= Kp * 2^14 <= 2^30, so
+  // float(a) * (1/m) is within 1 of a/m (float(a) is off by at most 32,
+  // 1/m by a relative 2^-24), and one correction each way is enough.
+  const float inv = 1.0f / (float)mod;
   for_each_acc<Elem::I8, BM, BN, WM, WN, MI>(
       acc, p.N, p.M, tr * BM, tc * BN, [&](int n, int m, int a) {
-        int u = a % mod;
-        if (u < 0) u += mod;
+        const int q = (int)__builtin_floorf((float)a * inv);
+        int u = a - q * mod;
+        u += u < 0 ? mod : 0;
+        u -= u >= mod ? mod : 0;
         R[(int64_t)n * p.M + m] = (uint8_t)u;
       });
 }

@@ -407,17 +407,18 @@ hcblasStatus_t run_reduce(hcblasHandle_t h, int64_t n, int64_t batch,
   const bool contiguous = incx == 1 && (!Red::two || incy == 1);
   const T *yy = Red::two ? y : x;
   if (contiguous && aligned16(x) && (!Red::two || aligned16(y))) {
+    // Same bytes in flight per lane for one stream (asum) as for two (dot).
+    constexpr int U = Red::two ? kReduceUnroll : 2 * kReduceUnroll;
     const int64_t nvec = flat / VN;
     const int grid = (int)std::max<int64_t>(
-        1, std::min<int64_t>(cdiv(nvec, kThreads * kReduceUnroll),
-                             kReduceGrid));
+        1, std::min<int64_t>(cdiv(nvec, kThreads * U), kReduceGrid));
     const int64_t bytes = flat * (int64_t)sizeof(T) * (Red::two ? 2 : 1);
     if (bytes >= kStreamBytes)
-      reduce_vec_kernel<T, Red, true, kReduceUnroll>
+      reduce_vec_kernel<T, Red, true, U>
           <<<grid, kThreads, 0, h->stream>>>(flat, x, yy, h->d_partials,
                                              h->d_ticket, dres, red);
     else
-      reduce_vec_kernel<T, Red, false, kReduceUnroll>
+      reduce_vec_kernel<T, Red, false, U>
           <<<grid, kThreads, 0, h->stream>>>(flat, x, yy, h->d_partials,
                                              h->d_ticket, dres, red);
   } else if (contiguous) {

@@ -118,3 +118,48 @@ def test_python_api_boxes_scalars():
     assert api.hcblasZgemm(None, 0, 0, 1, 1, 1, 1 + 2j, 0, 1, 0, 1, 0j, 0, 1) == 1
     with pytest.raises(TypeError):
         api.hcblasSgemm(None, 0, 0)
+
+
+def test_torch_extension_registers_ops():
+    """The optional libtorch layer loads against libhcblas.so and registers
+    its two differentiable ops (no GPU call)."""
+    import os
+    from hcblas_amd import torch_ext
+    if not os.path.exists(torch_ext.LIB_PATH):
+        pytest.skip("libhcblas_torch.so not built")
+    import torch
+    torch_ext.load()
+    schema = str(torch.ops.hcblas.gemm_batched.default._schema)
+    assert "Tensor? C" in schema and "bool transB" in schema
+    assert "float alpha" in str(torch.ops.hcblas.ger.default._schema)
+
+
+def test_hipblas_shim_exports_and_null_handle():
+    """The hipBLAS-compatible library exports every routine of
+    include/hipblas_hcblas.h, and a null handle maps to
+    HIPBLAS_STATUS_NOT_INITIALIZED (hipblas.cpp:107-126)."""
+    from hcblas_amd import hipblas_shim
+    lib = hipblas_shim.load()
+    protos = _abi.parse_header(_abi.SHIM_HEADER, "hipblasStatus_t")
+    assert len(protos) >= 50
+    out = subprocess.run(["nm", "-D", "--defined-only", hipblas_shim.LIB_PATH],
+                         capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r"\sT\s(\S+)", out))
+    assert not [n for n in protos if n not in exported]
+    handle_less = {"hipblasCreate", "hipblasSetVector", "hipblasGetVector",
+                   "hipblasSetMatrix", "hipblasGetMatrix"}
+    for name, proto in protos.items():
+        if name in handle_less:
+            continue
+        args = []
+        for ctype, _ in proto.args:
+            t = _abi._ctype(ctype)
+            if ctype == "hipblasOperation_t":
+                args.append(hipblas_shim.OP_N)
+            else:
+                args.append(None if t is ctypes.c_void_p else 1)
+        assert getattr(lib, name)(*args) == 1, name
+    # an unknown operation code is an invalid value, not an exception
+    assert lib.hipblasSgemm(None, 0, hipblas_shim.OP_N, 1, 1, 1, None, None, 1,
+                            None, 1, None, None, 1) == 3
+    assert lib.hipblasCreate(None) == 3
