@@ -30,9 +30,6 @@
 //     a GROUP_M-grouped tile order (A and B panels stay in that XCD's L2).
 #pragma once
 
-#ifndef HCB_EARLY_STORE
-#define HCB_EARLY_STORE 0
-#endif
 
 #include "common.hpp"
 
@@ -484,7 +481,6 @@ __device__ __forceinline__ void gemm_mainloop(
   auto buf = [&](int kt) { return lds + (kt & 1) * (A_BYTES + B_BYTES); };
 
   if (nk == 0) return;
-#if HCB_EARLY_STORE
   // Pipeline: the registers hold step kt+1 while step kt computes. At the
   // top of step kt they are written to the other LDS buffer (free: every
   // wave passed the previous barrier) and refilled with step kt+2, so the
@@ -515,42 +511,6 @@ __device__ __forceinline__ void gemm_mainloop(
     compute(buf(kt));
     __syncthreads();
   }
-#else
-  la.load_step(A, lda, m0, 0, M, K, nfull > 0);
-  lb.load_step(B, ldb, n0, 0, N, K, nfull > 0);
-  la.store(lds);
-  lb.store(lds + A_BYTES);
-  __syncthreads();
-
-  // Steady state: the step being fetched is a full one, so the loads are
-  // straight-line code and their registers are first read by the LDS
-  // stores after the MFMAs.
-  int kt = 0;
-  for (; kt + 1 < nfull; ++kt) {
-    la.load_step(A, lda, m0, (kt + 1) * BK, M, K, true);
-    lb.load_step(B, ldb, n0, (kt + 1) * BK, N, K, true);
-    // Keep the loads ahead of every MFMA of the step.
-    __builtin_amdgcn_sched_barrier(0);
-    compute(buf(kt));
-    la.store(buf(kt + 1));
-    lb.store(buf(kt + 1) + A_BYTES);
-    __syncthreads();
-  }
-  // The last full step and the K tail (at most two steps).
-  for (; kt < nk; ++kt) {
-    const bool more = kt + 1 < nk;
-    if (more) {
-      la.load(A, lda, m0, (kt + 1) * BK, M, K);
-      lb.load(B, ldb, n0, (kt + 1) * BK, N, K);
-    }
-    compute(buf(kt));
-    if (more) {
-      la.store(buf(kt + 1));
-      lb.store(buf(kt + 1) + A_BYTES);
-    }
-    __syncthreads();
-  }
-#endif
 }
 
 // Calls f(row, col, value) for every accumulator value of this lane that

@@ -92,7 +92,9 @@ hcblasStatus_t launch_gemm(hipStream_t stream, int num_cus, GemmParams p,
       return launch_tiles<E, OutT, 128, 64, 16, 2, 4>(stream, p, a_k, b_k,
                                                       vload, batch);
   } else {
-    if (tiles(256, 256) >= num_cus)
+    // (the element-wise loaders of unaligned operands do not fit the 256x256
+    // tile's registers: they take the 128x128 tile)
+    if (vload && tiles(256, 256) >= num_cus)
       return launch_tiles<E, OutT, 256, 256, TileCfg<E>::BK_BIG, 2, 4, 32>(
           stream, p, a_k, b_k, vload, batch);
     if (tiles(128, 128) >= num_cus)

@@ -161,7 +161,6 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_planes_kernel(
     }
   };
 
-#if HCB_EARLY_STORE
   // Registers hold tile t+1 while tile t computes; they are written to the
   // other LDS buffer at the top of the step and refilled with tile t+2.
   int lg = 0, lkt = 0;  // cursor of the tile in the registers
@@ -183,24 +182,6 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_planes_kernel(
     finish_tile();
     __syncthreads();
   }
-#else
-  int lg = 0, lkt = 0;  // cursor of the tile being loaded
-  load_tile(0, 0);
-  store(0);
-  __syncthreads();
-  for (int t = 0; t < total; ++t) {
-    const bool more = t + 1 < total;
-    if (more) {
-      advance(lg, lkt);
-      load_tile(lg, lkt);
-    }
-    __builtin_amdgcn_sched_barrier(0);
-    compute(buf(t));
-    if (more) store(t + 1);
-    finish_tile();
-    __syncthreads();
-  }
-#endif
 
   OutT *C = static_cast<OutT *>(p.C);
 #pragma unroll

@@ -142,8 +142,9 @@ hcblasStatus_t hcblasSetOzakiSlices(hcblasHandle_t handle, int slices);
 
 /* Number of moduli for the OZAKI_CRT path, 8..20, or 0 (the default) to
  * pick, per K, the fewest moduli whose operand precision matches the plain
- * FP64 kernel's error: b >= 54 - log2(K)/2 bits, i.e. 14 moduli (48 bits)
- * at K = 8192 (no reference counterpart). */
+ * FP64 kernel's error, also for inputs with a wide exponent spread:
+ * b >= 59 - log2(K)/2 bits, i.e. 16 moduli (55 bits) at K = 8192 (no
+ * reference counterpart). */
 hcblasStatus_t hcblasSetOzakiModuli(hcblasHandle_t handle, int moduli);
 
 /* ------------------------------------------------------------------------ */
