@@ -248,16 +248,6 @@ constexpr int inv_mod(int a, int m) {
   return t < 0 ? t + m : t;
 }
 
-// inv[t][s] = m_s^-1 mod m_t for s < t (Garner's constants).
-struct CrtInverses {
-  int inv[kMaxModuli][kMaxModuli];
-  constexpr CrtInverses() : inv() {
-    for (int t = 0; t < kMaxModuli; ++t)
-      for (int s = 0; s < t; ++s)
-        inv[t][s] = inv_mod(kModuli[s] % kModuli[t], kModuli[t]);
-  }
-};
-
 template <int T>
 __device__ __forceinline__ constexpr int modulus() {
   constexpr int m[kMaxModuli] = {256, 255, 253, 251, 247, 241, 239,
@@ -265,33 +255,56 @@ __device__ __forceinline__ constexpr int modulus() {
                                  197, 193, 191, 181, 179, 173};
   return m[T];
 }
-template <int T, int S>
-__device__ __forceinline__ constexpr int garner_inv() {
-  constexpr CrtInverses tab{};
-  return tab.inv[T][S];
-}
 
-// Symmetric residue of v modulo M: [-M/2, (M-1)/2].
+// s mod M for 0 <= s < 2^30 without an integer division: float(s) is off by
+// at most 32 and 1/M by a relative 2^-24, so the float quotient is within 1
+// of s/M and one correction each way suffices.
 template <int M>
-__device__ __forceinline__ int sym_mod(long long v) {
-  int q = (int)(v % M);
-  if (q > (M - 1) / 2)
-    q -= M;
-  else if (q < -(M / 2))
-    q += M;
-  return q;
+__device__ __forceinline__ int mod_small(int s) {
+  constexpr float inv = 1.0f / (float)M;
+  const int q = (int)__builtin_floorf((float)s * inv);
+  int r = s - q * M;
+  r += r < 0 ? M : 0;
+  r -= r >= M ? M : 0;
+  return r;
 }
 
+// Symmetric residue of v = sign * (l2 2^42 + l1 2^21 + l0) (limbs < 2^21)
+// modulo M, as the byte of an int8 in [-M/2, (M-1)/2].
+template <int M>
+__device__ __forceinline__ uint32_t residue_byte(int l2, int l1, int l0,
+                                                 bool neg) {
+  constexpr int c21 = (int)((1ll << 21) % M);
+  constexpr int c42 = (int)((1ll << 42) % M);
+  int q = mod_small<M>(l2 * c42 + l1 * c21 + l0);  // < 2^30
+  if (neg && q != 0) q = M - q;
+  if (q > (M - 1) / 2) q -= M;
+  return (uint32_t)(q & 0xff);
+}
+
+struct Limbs {
+  int l2[4], l1[4], l0[4];
+  bool neg[4];
+};
+
+// Four consecutive k of one row: one packed dword per modulus plane.
 template <int NMOD, int T = 0>
-__device__ __forceinline__ void write_residues(long long v, int8_t *row,
-                                               int64_t Kp) {
+__device__ __forceinline__ void write_residues4(const Limbs &L, int8_t *row,
+                                                int64_t Kp) {
   if constexpr (T < NMOD) {
-    row[(int64_t)T * Kp] = (int8_t)sym_mod<modulus<T>()>(v);
-    write_residues<NMOD, T + 1>(v, row, Kp);
+    constexpr int M = modulus<T>();
+    uint32_t w = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      w |= residue_byte<M>(L.l2[j], L.l1[j], L.l0[j], L.neg[j]) << (8 * j);
+    *reinterpret_cast<uint32_t *>(row + (int64_t)T * Kp) = w;
+    write_residues4<NMOD, T + 1>(L, row, Kp);
   }
 }
 
-// Residue planes P[r][t][0..Kp) of rint(X(r, k) * 2^(bits - e_r)).
+// Residue planes P[r][t][0..Kp) of rint(X(r, k) * 2^(bits - e_r)). A
+// workgroup stages a 32 x 32 tile; each thread converts 4 consecutive k of
+// one row and writes one dword per plane (Kp is a multiple of 32).
 template <int NMOD>
 __global__ __launch_bounds__(256) void crt_residue_kernel(
     int R, int K, const double *X, int64_t ld, bool kcont, const int *exps,
@@ -299,15 +312,29 @@ __global__ __launch_bounds__(256) void crt_residue_kernel(
   __shared__ TileStage<double> st;
   const int r0 = blockIdx.y * 32, k0 = blockIdx.x * 32;
   st.load(X, ld, kcont, R, K, r0, k0);
-  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-  for (int i = ty; i < 32; i += 8) {
-    const int r = r0 + i, k = k0 + tx;
-    if (r >= R || k >= Kp) continue;
-    const int e = exps[r];
-    const double v =
-        e == INT_MIN ? 0.0 : __builtin_rint(__builtin_ldexp(st.v[i][tx], bits - e));
-    write_residues<NMOD>((long long)v, P + (int64_t)r * NMOD * Kp + k, Kp);
+  const int i = threadIdx.x >> 3, kq = (threadIdx.x & 7) * 4;
+  const int r = r0 + i, k = k0 + kq;
+  if (r >= R) return;
+  const int e = exps[r];
+  Limbs L;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const double v = e == INT_MIN
+                         ? 0.0
+                         : __builtin_rint(
+                               __builtin_ldexp(st.v[i][kq + j], bits - e));
+    // |v| <= 2^62 is an integer; split it exactly into three 21-bit limbs.
+    const double a = __builtin_fabs(v);
+    const double a2 = __builtin_floor(a * 0x1p-42);
+    const double rem = a - a2 * 0x1p42;
+    const double a1 = __builtin_floor(rem * 0x1p-21);
+    const double a0 = rem - a1 * 0x1p21;
+    L.l2[j] = (int)a2;
+    L.l1[j] = (int)a1;
+    L.l0[j] = (int)a0;
+    L.neg[j] = v < 0.0;
   }
+  write_residues4<NMOD>(L, P + (int64_t)r * NMOD * Kp + k, Kp);
 }
 
 struct CrtParams {
@@ -351,43 +378,114 @@ __global__ __launch_bounds__(64 * WM * WN) void crt_gemm_kernel(CrtParams p) {
       });
 }
 
-// Garner digit t (symmetric) from residue u_t and the digits before it.
-template <int T, int S = 0>
-__device__ __forceinline__ int garner_step(int x, const int *v) {
+// Reconstruction. The moduli are taken in groups of three (the last group
+// may hold one or two); a group's residues combine exactly, in 32-bit
+// integers, into one residue modulo the group product SM_g < 2^24 (Garner
+// within the group). Garner's algorithm with symmetric digits then runs over
+// the super-moduli SM_g in FP64 arithmetic, where every intermediate is an
+// integer below 2^50 and so exact, and the mixed-radix value is evaluated by
+// Horner's rule in double-double.
+constexpr long long inv_mod64(long long a, long long m) {
+  long long t = 0, nt = 1, r = m, nr = a % m;
+  while (nr != 0) {
+    const long long q = r / nr;
+    const long long t2 = t - q * nt;
+    t = nt;
+    nt = t2;
+    const long long r2 = r - q * nr;
+    r = nr;
+    nr = r2;
+  }
+  return t < 0 ? t + m : t;
+}
+template <int NMOD, int G>
+__device__ __forceinline__ constexpr long long super_modulus() {
+  constexpr int i0 = 3 * G;
+  long long p = modulus<i0>();
+  if constexpr (i0 + 1 < NMOD) p *= modulus<i0 + 1>();
+  if constexpr (i0 + 2 < NMOD) p *= modulus<i0 + 2>();
+  return p;
+}
+
+// Residues u of the moduli of group G -> residue modulo SM_G, in [0, SM_G).
+template <int NMOD, int G>
+__device__ __forceinline__ int group_residue(const int *u) {
+  constexpr int i0 = 3 * G;
+  constexpr int m0 = modulus<i0>();
+  if constexpr (i0 + 1 >= NMOD) {
+    return u[i0];
+  } else {
+    constexpr int m1 = modulus<i0 + 1>();
+    constexpr int inv01 = inv_mod(m0 % m1, m1);
+    // u1 - u0 + 2 m1 > 0 and < 2^10, times inv01 < 2^18
+    const int d1 = mod_small<m1>((u[i0 + 1] - u[i0] + 2 * m1) * inv01);
+    const int part = u[i0] + m0 * d1;  // residue modulo m0 m1
+    if constexpr (i0 + 2 >= NMOD) {
+      return part;
+    } else {
+      constexpr int m2 = modulus<i0 + 2>();
+      constexpr int inv2 = inv_mod((m0 * m1) % m2, m2);
+      constexpr int c = m2 * ((65536 + m2 - 1) / m2);  // >= m0 m1
+      const int d2 =
+          mod_small<m2>(mod_small<m2>(u[i0 + 2] - part + c) * inv2);
+      return part + m0 * m1 * d2;
+    }
+  }
+}
+
+// y mod SM for an integer-valued 0 <= y < 2^53 (rcp = 1/SM).
+__device__ __forceinline__ double dmod(double y, double SM, double rcp) {
+  const double q = __builtin_floor(y * rcp);
+  double r = __builtin_fma(-q, SM, y);  // exact: an integer of |r| < 2 SM
+  r += r < 0.0 ? SM : 0.0;
+  r -= r >= SM ? SM : 0.0;
+  return r;
+}
+
+// Garner digit T over the super-moduli (symmetric digits V[0..T)).
+template <int NMOD, int T, int S = 0>
+__device__ __forceinline__ double garner_super_step(double x, const double *V) {
   if constexpr (S < T) {
-    constexpr int M = modulus<T>();
-    x = ((x - v[S] + 2 * M) * garner_inv<T, S>()) % M;
-    return garner_step<T, S + 1>(x, v);
+    constexpr long long SMt = super_modulus<NMOD, T>();
+    constexpr long long SMs = super_modulus<NMOD, S>();
+    constexpr double inv = (double)inv_mod64(SMs % SMt, SMt);
+    // a multiple of SM_t above 2^23 > |V[S]|
+    constexpr double c = (double)(SMt * (((1ll << 23) + SMt - 1) / SMt));
+    const double y = (x - V[S] + c) * inv;  // integer < 2^50: exact
+    x = dmod(y, (double)SMt, 1.0 / (double)SMt);
+    return garner_super_step<NMOD, T, S + 1>(x, V);
   } else {
     return x;
   }
 }
-template <int NMOD, int T = 0>
-__device__ __forceinline__ void garner_digits(const int *u, int *v) {
-  if constexpr (T < NMOD) {
-    constexpr int M = modulus<T>();
-    const int x = garner_step<T>(u[T], v);
-    v[T] = x > (M - 1) / 2 ? x - M : x;
-    garner_digits<NMOD, T + 1>(u, v);
+template <int NMOD, int NG, int T = 0>
+__device__ __forceinline__ void garner_super(const int *u, double *V) {
+  if constexpr (T < NG) {
+    constexpr long long SM = super_modulus<NMOD, T>();
+    const double x =
+        garner_super_step<NMOD, T>((double)group_residue<NMOD, T>(u), V);
+    V[T] = x > (double)((SM - 1) / 2) ? x - (double)SM : x;
+    garner_super<NMOD, NG, T + 1>(u, V);
   }
 }
-// Mixed-radix value sum_t v_t * prod_{s<t} m_s by Horner's rule in
+
+// Mixed-radix value sum_t V_t * prod_{s<t} SM_s by Horner's rule in
 // double-double (about 106 bits; the exact value has up to ~126), so the
 // returned double is the correctly rounded value except within 2^-105 of a
 // rounding boundary, and exactly representable results come out exact.
-template <int NMOD, int T>
-__device__ __forceinline__ void horner_dd(const int *v, double &hi,
+template <int NMOD, int NG, int T>
+__device__ __forceinline__ void horner_dd(const double *V, double &hi,
                                           double &lo) {
-  if constexpr (T == NMOD - 1) {
-    hi = (double)v[T];
+  if constexpr (T == NG - 1) {
+    hi = V[T];
     lo = 0.0;
   } else {
 #pragma clang fp contract(off)
-    horner_dd<NMOD, T + 1>(v, hi, lo);
-    const double m = (double)modulus<T>();
+    horner_dd<NMOD, NG, T + 1>(V, hi, lo);
+    const double m = (double)super_modulus<NMOD, T>();
     const double p = hi * m;
     const double e = __builtin_fma(hi, m, -p);  // hi*m = p + e exactly
-    const double vt = (double)v[T];
+    const double vt = V[T];
     const double s = p + vt;                    // two-sum: p + vt = s + t
     const double bb = s - p;
     const double t = (p - (s - bb)) + (vt - bb);
@@ -396,35 +494,63 @@ __device__ __forceinline__ void horner_dd(const int *v, double &hi,
     lo = l - (hi - s);
   }
 }
+
+// One output element from its residues.
 template <int NMOD>
-__device__ __forceinline__ double horner(const int *v) {
+__device__ __forceinline__ double crt_value(const int *u) {
+  constexpr int NG = (NMOD + 2) / 3;
+  double V[NG];
+  garner_super<NMOD, NG>(u, V);
   double hi, lo;
-  horner_dd<NMOD, 0>(v, hi, lo);
+  horner_dd<NMOD, NG, 0>(V, hi, lo);
   return hi + lo;
 }
 
-// C(m, n) = alpha * 2^(e_m + f_n - shift) * CRT(R[.][n][m]) + beta * C(m, n).
-template <int NMOD>
-__global__ __launch_bounds__(256) void crt_combine_kernel(
-    int M, int N, const uint8_t *R, const int *ea, const int *eb, int shift,
-    double alpha, double beta, double *C, int64_t ldc) {
-  const int64_t MN = (int64_t)M * N;
-  const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (idx >= MN) return;
-  const int m = (int)(idx % M), n = (int)(idx / M);
-  int u[NMOD], v[NMOD];
-#pragma unroll
-  for (int t = 0; t < NMOD; ++t) u[t] = R[t * MN + idx];
-  garner_digits<NMOD>(u, v);
-  const double x = horner<NMOD>(v);
-  const int re = ea[m], ce = eb[n];
+__device__ __forceinline__ void crt_store(double x, int re, int ce, int shift,
+                                          double alpha, double beta,
+                                          double *dst) {
   const double val = (re == INT_MIN || ce == INT_MIN)
                          ? 0.0
                          : __builtin_ldexp(x, re + ce - shift);
-  double *dst = C + (int64_t)n * ldc + m;
   double o = alpha * val;
   if (beta != 0.0) o = __builtin_fma(beta, *dst, o);
   *dst = o;
+}
+
+// C(m, n) = alpha * 2^(e_m + f_n - shift) * CRT(R[.][n][m]) + beta * C(m, n).
+// VEC4 (M % 4 == 0): a thread takes 4 consecutive m of one column and reads
+// one dword per residue plane.
+template <int NMOD, bool VEC4>
+__global__ __launch_bounds__(256) void crt_combine_kernel(
+    int M, int N, const uint8_t *R, const int *ea, const int *eb, int shift,
+    double alpha, double beta, double *C, int64_t ldc) {
+#pragma clang fp contract(off)
+  const int64_t MN = (int64_t)M * N;
+  constexpr int W = VEC4 ? 4 : 1;
+  const int64_t idx = ((int64_t)blockIdx.x * 256 + threadIdx.x) * W;
+  if (idx >= MN) return;
+  const int m = (int)(idx % M), n = (int)(idx / M);
+  const int ce = eb[n];
+  double *dst = C + (int64_t)n * ldc + m;
+  if constexpr (VEC4) {
+    uint32_t w[NMOD];
+#pragma unroll
+    for (int t = 0; t < NMOD; ++t)
+      w[t] = *reinterpret_cast<const uint32_t *>(R + t * MN + idx);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      int u[NMOD];
+#pragma unroll
+      for (int t = 0; t < NMOD; ++t) u[t] = (w[t] >> (8 * j)) & 0xff;
+      crt_store(crt_value<NMOD>(u), ea[m + j], ce, shift, alpha, beta,
+                dst + j);
+    }
+  } else {
+    int u[NMOD];
+#pragma unroll
+    for (int t = 0; t < NMOD; ++t) u[t] = R[t * MN + idx];
+    crt_store(crt_value<NMOD>(u), ea[m], ce, shift, alpha, beta, dst);
+  }
 }
 
 template <int NMOD>
@@ -439,9 +565,13 @@ template <int NMOD>
 void launch_crt_combine(hipStream_t s, int M, int N, const uint8_t *R,
                         const int *ea, const int *eb, int shift, double alpha,
                         double beta, double *C, int64_t ldc) {
-  crt_combine_kernel<NMOD>
-      <<<(unsigned)cdiv((int64_t)M * N, 256), 256, 0, s>>>(
-          M, N, R, ea, eb, shift, alpha, beta, C, ldc);
+  const int64_t MN = (int64_t)M * N;
+  if (M % 4 == 0)
+    crt_combine_kernel<NMOD, true><<<(unsigned)cdiv(MN / 4, 256), 256, 0, s>>>(
+        M, N, R, ea, eb, shift, alpha, beta, C, ldc);
+  else
+    crt_combine_kernel<NMOD, false><<<(unsigned)cdiv(MN, 256), 256, 0, s>>>(
+        M, N, R, ea, eb, shift, alpha, beta, C, ldc);
 }
 
 // Runtime modulus count -> template instance (8..20).
