@@ -12,8 +12,8 @@
 //   col-major T, row-major N  ->  "column" kernel: y[c] = sum_r S[r,c] x[r]
 // Both kernels stream S once with 16-byte loads and split the reduction
 // across workgroups when the output alone cannot fill 256 CUs; split partials
-// go to a per-handle workspace and the last workgroup of each output block
-// (ticket counter) sums them in split order, so results are deterministic.
+// go to a per-handle workspace and a small finish kernel on the same stream
+// sums them in split order, so results are deterministic.
 //
 // GER updates S[r,c] = fma(v[r], alpha*w[c], S[r,c]) where w is the vector
 // that indexes storage columns; with v = x, w = y for column-major and
