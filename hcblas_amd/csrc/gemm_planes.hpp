@@ -6,13 +6,7 @@
 // plane a0_g with the window of np_g consecutive B planes starting at b0_g:
 //     G_g(m,n) = sum_{j<np_g} sum_k Ap[m][a0_g+j][k] * Bp[n][b0_g+j][k]
 // which is one long-K GEMM (K = np_g*Kp) because the windows are contiguous.
-// Groups collect the products of equal magnitude:
-//   - BF16 split SGEMM: A planes (hi, mid, lo), B planes (lo, mid, hi);
-//     groups {A[0..3) x B[0..3)} = hi*lo + mid*mid + lo*hi,
-//            {A[0..2) x B[1..3)} = hi*mid + mid*hi,
-//            {A[0..1) x B[2..3)} = hi*hi,
-//     accumulated smallest first into one FP32 MFMA accumulator.
-//   - Ozaki INT8 DGEMM: A planes = slices 1..s, B planes = slices s..1; group
+// Groups collect the products of equal magnitude. Ozaki INT8 DGEMM: A planes = slices 1..s, B planes = slices s..1; group
 //     d (t+u = d) is A[0..d-1) x B[s-d+1..s), summed exactly in INT32 and
 //     added to an FP64 accumulator with weight 2^(-7(d-2)); the epilogue
 //     applies the per-row / per-column power-of-two scales.

@@ -3,8 +3,10 @@
 //     accumulation (six products: hi*hi, hi*mid, mid*hi, hi*lo, mid*mid,
 //     lo*hi);
 //   - DGEMM / ZGEMM as Ozaki exact INT8 slices on INT8 MFMA.
-// Both pack the operands once into k-contiguous planes and run the
-// plane-group kernel of gemm_planes.hpp.
+// The BF16 split packs both operands into blocked planes and runs the fused
+// kernel below (all six products from one set of fragments per K step); the
+// INT8 slice form packs k-contiguous planes for the plane-group kernel of
+// gemm_planes.hpp.
 //
 // BF16 split: x = hi + mid + lo + d with hi = bf16(x), mid = bf16(x - hi),
 // lo = bf16(x - hi - mid) (round-to-nearest-even; the residuals are exact in
@@ -76,29 +78,255 @@ struct TileStage {
   }
 };
 
-// BF16 three-way split into planes P[r][q][0..Kp), q = plane index
-// (hi, mid, lo), or (lo, mid, hi) when reverse. Covers k < Kp (zeros past K).
-__global__ __launch_bounds__(256) void split_bf16_kernel(
-    int R, int K, const float *X, int64_t ld, bool kcont, uint16_t *P,
-    int64_t Kp, bool reverse) {
-  __shared__ TileStage<float> st;
-  const int r0 = blockIdx.y * 32, k0 = blockIdx.x * 32;
-  st.load(X, ld, kcont, R, K, r0, k0);
-  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-  for (int i = ty; i < 32; i += 8) {
-    const int r = r0 + i, k = k0 + tx;
-    if (r >= R || k >= Kp) continue;
-    const float x = st.v[i][tx];
-    const uint16_t h = f32_to_bf16_rne(x);
-    const float r1 = x - bf16_to_f32(h);
-    const uint16_t m = f32_to_bf16_rne(r1);
-    const float r2 = r1 - bf16_to_f32(m);
-    const uint16_t l = f32_to_bf16_rne(r2);
-    uint16_t *row = P + (int64_t)r * 3 * Kp + k;
-    row[(reverse ? 2 : 0) * Kp] = h;
-    row[1 * Kp] = m;
-    row[(reverse ? 0 : 2) * Kp] = l;
+// ---------------------------------------------------------------------------
+// Fused BF16x3 split GEMM.
+//
+// Planes are packed per 16-wide K block (one 32x32x16 MFMA chunk):
+//   P[kb][r][q][0..16)   q = 0 (hi), 1 (mid), 2 (lo),  kb = k / 16,
+// so the three planes of one K block of a tile of rows are one contiguous run
+// of rows * 96 bytes. K is padded with zeros to a whole number of blocks.
+constexpr int kS3K = 16;             // k per block
+constexpr int kS3Row = 3 * kS3K;     // elements per row per block
+
+// Splits x into hi + mid + lo (each the BF16 rounding of the remaining
+// residual). Non-finite x keeps hi = bf16(x) (NaN stays NaN) and zero
+// residual planes, so Inf and NaN propagate as in FP32.
+__device__ __forceinline__ void split3(float x, uint16_t &h, uint16_t &m,
+                                       uint16_t &l) {
+  const uint32_t u = __float_as_uint(x);
+  if ((u & 0x7F800000u) == 0x7F800000u) {
+    h = (uint16_t)((u >> 16) | ((u & 0x7FFFFFu) ? 0x40u : 0u));
+    m = l = 0;
+    return;
   }
+  h = f32_to_bf16_rne(x);
+  const float r1 = x - bf16_to_f32(h);
+  m = f32_to_bf16_rne(r1);
+  const float r2 = r1 - bf16_to_f32(m);
+  l = f32_to_bf16_rne(r2);
+}
+
+// Packs an R x K operand view (kcont: element (r,k) at X[k + r*ld], else at
+// X[r + k*ld]) into the blocked planes above. A workgroup stages a 64 (r) x
+// 32 (k) tile through LDS, reading along the source's contiguous dimension;
+// each thread then splits 8 consecutive k of one row and writes one 16-byte
+// vector per plane.
+__global__ __launch_bounds__(256) void split3_pack_kernel(
+    int R, int K, const float *__restrict__ X, int64_t ld, bool kcont,
+    uint16_t *__restrict__ P, int nkb) {
+  __shared__ float t[64][33];
+  const int r0 = blockIdx.y * 64, k0 = blockIdx.x * 32;
+  const int tid = threadIdx.x;
+  if (kcont) {
+    const int tx = tid & 31;
+    for (int i = tid >> 5; i < 64; i += 8) {
+      const int r = r0 + i, k = k0 + tx;
+      t[i][tx] = (r < R && k < K) ? X[(int64_t)r * ld + k] : 0.f;
+    }
+  } else {
+    const int tx = tid & 63;
+    for (int i = tid >> 6; i < 32; i += 4) {
+      const int r = r0 + tx, k = k0 + i;
+      t[tx][i] = (r < R && k < K) ? X[(int64_t)k * ld + r] : 0.f;
+    }
+  }
+  __syncthreads();
+  const int row = tid >> 2, kg = tid & 3;
+  const int r = r0 + row;
+  const int kb = (k0 >> 4) + (kg >> 1);
+  if (r >= R || kb >= nkb) return;
+  uint32_t hv[4], mv[4], lv[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    uint16_t h0, m0, l0, h1, m1, l1;
+    split3(t[row][kg * 8 + 2 * e], h0, m0, l0);
+    split3(t[row][kg * 8 + 2 * e + 1], h1, m1, l1);
+    hv[e] = (uint32_t)h0 | ((uint32_t)h1 << 16);
+    mv[e] = (uint32_t)m0 | ((uint32_t)m1 << 16);
+    lv[e] = (uint32_t)l0 | ((uint32_t)l1 << 16);
+  }
+  uint16_t *dst = P + ((int64_t)kb * R + r) * kS3Row + (kg & 1) * 8;
+  *reinterpret_cast<vec4u *>(dst) = vec4u{hv[0], hv[1], hv[2], hv[3]};
+  *reinterpret_cast<vec4u *>(dst + kS3K) = vec4u{mv[0], mv[1], mv[2], mv[3]};
+  *reinterpret_cast<vec4u *>(dst + 2 * kS3K) =
+      vec4u{lv[0], lv[1], lv[2], lv[3]};
+}
+
+struct Split3Params {
+  int M, N, nkb;
+  const uint16_t *Ap;  // [nkb][M][3][16]
+  const uint16_t *Bp;  // [nkb][N][3][16]
+  float *C;            // column-major M x N
+  int64_t ldc;
+  float alpha, beta;
+  int tiles_m, tiles_n;
+};
+
+// Global -> register -> LDS loader of the SKB K blocks of one step of a
+// ROWS-row tile. LDS tile: [ROWS][SKB][3][16] plus 16 bytes per row (the row
+// stride of 112 or 208 bytes spreads the 16-byte fragment reads of 8
+// consecutive rows over distinct banks). Rows past R read row r0 instead;
+// they only feed outputs that are never stored.
+template <int ROWS, int SKB, int NTH>
+struct Split3Loader {
+  static constexpr int VPB = ROWS * 6;       // 16-byte vectors per block
+  static constexpr int NV = VPB * SKB;
+  static constexpr int PER = (NV + NTH - 1) / NTH;
+  static constexpr int ROWB = SKB * 96 + 16;
+  vec4u reg[PER];
+
+  __device__ __forceinline__ void load(const uint16_t *__restrict__ P, int R,
+                                       int r0, int kb) {
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int v = threadIdx.x + i * NTH;
+      if (NV % NTH != 0 && v >= NV) continue;
+      const int b = v / VPB, w = v % VPB;
+      const int row = w / 6, c = w % 6;
+      const int gr = r0 + row < R ? r0 + row : r0;
+      reg[i] = *reinterpret_cast<const vec4u *>(
+          P + ((int64_t)(kb + b) * R + gr) * kS3Row + c * 8);
+    }
+  }
+  __device__ __forceinline__ void store(char *lds) const {
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int v = threadIdx.x + i * NTH;
+      if (NV % NTH != 0 && v >= NV) continue;
+      const int b = v / VPB, w = v % VPB;
+      const int row = w / 6, c = w % 6;
+      *reinterpret_cast<vec4u *>(lds + row * ROWB + b * 96 + c * 16) = reg[i];
+    }
+  }
+};
+
+// C = alpha * A'B' + beta * C from the packed planes, all six products of a
+// K step issued from one set of fragments. hi*hi accumulates alone; the five
+// smaller products (hi*lo, mid*mid, lo*hi, hi*mid, mid*hi, in that order)
+// go to a second accumulator, so the roundings of the dominant sum are the
+// same as those of one 32x32x16 MFMA chain and the small terms add only
+// errors of their own (2^-8 smaller) magnitude; the two meet in the epilogue.
+// Each wave owns a WTM x WTN sub-tile of 32x32 blocks computed transposed
+// (MFMA rows = n, columns = m) so that the 32 lanes of a block store 32
+// consecutive m of a column-major C column.
+template <int BM, int BN, int WM, int WN, int SKB>
+__global__ __launch_bounds__(64 * WM * WN) void gemm_split3_kernel(
+    Split3Params p) {
+  constexpr int NTH = 64 * WM * WN;
+  constexpr int WTM = BM / WM, WTN = BN / WN;
+  constexpr int FM = WTM / 32, FN = WTN / 32;
+  static_assert(FM * 32 == WTM && FN * 32 == WTN, "wave tile");
+  using LA = Split3Loader<BM, SKB, NTH>;
+  using LB = Split3Loader<BN, SKB, NTH>;
+  constexpr int ROWB = LA::ROWB;
+  constexpr int A_BYTES = BM * ROWB, B_BYTES = BN * ROWB;
+  __shared__ __attribute__((aligned(16))) char lds[2 * (A_BYTES + B_BYTES)];
+
+  int tile_m, tile_n;
+  tile_coords(blockIdx.x, p.tiles_m, p.tiles_n, tile_m, tile_n);
+  const int m0 = tile_m * BM, n0 = tile_n * BN;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int wm = wave / WN, wn = wave % WN;
+  const int l32 = lane & 31, half = lane >> 5;
+
+  floatx16 big[FM][FN], small[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) big[i][j] = small[i][j] = floatx16{};
+
+  const int nsteps = p.nkb / SKB;  // nkb is a multiple of SKB
+  LA la;
+  LB lb;
+  auto load = [&](int s) {
+    la.load(p.Ap, p.M, m0, s * SKB);
+    lb.load(p.Bp, p.N, n0, s * SKB);
+  };
+  auto buf = [&](int s) { return lds + (s & 1) * (A_BYTES + B_BYTES); };
+  auto store = [&](int s) {
+    la.store(buf(s));
+    lb.store(buf(s) + A_BYTES);
+  };
+  auto compute = [&](const char *cur) {
+    const char *As = cur + (wm * WTM + l32) * ROWB + half * 16;
+    const char *Bs = cur + A_BYTES + (wn * WTN + l32) * ROWB + half * 16;
+#pragma unroll
+    for (int b = 0; b < SKB; ++b) {
+      vec4u af[3][FM], bf[3][FN];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+          af[q][i] = *reinterpret_cast<const vec4u *>(As + i * 32 * ROWB +
+                                                      b * 96 + q * 32);
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+          bf[q][j] = *reinterpret_cast<const vec4u *>(Bs + j * 32 * ROWB +
+                                                      b * 96 + q * 32);
+      }
+      // (A plane, B plane) of the small products, smallest first.
+      constexpr int qa[5] = {0, 1, 2, 0, 1};
+      constexpr int qb[5] = {2, 1, 0, 1, 0};
+#pragma unroll
+      for (int t = 0; t < 5; ++t)
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j)
+            small[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                bitcast<bf16x8>(bf[qb[t]][j]), bitcast<bf16x8>(af[qa[t]][i]),
+                small[i][j], 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+          big[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+              bitcast<bf16x8>(bf[0][j]), bitcast<bf16x8>(af[0][i]), big[i][j],
+              0, 0, 0);
+    }
+  };
+
+  // Registers hold step s+1 while step s computes; they are written to the
+  // other LDS buffer at the top of the step and refilled with step s+2.
+  load(0);
+  store(0);
+  if (nsteps > 1) load(1);
+  __syncthreads();
+  for (int s = 0; s < nsteps; ++s) {
+    if (s + 1 < nsteps) store(s + 1);
+    if (s + 2 < nsteps) load(s + 2);
+    __builtin_amdgcn_sched_barrier(0);
+    compute(buf(s));
+    __syncthreads();
+  }
+
+#pragma unroll
+  for (int i = 0; i < FM; ++i) {
+    const int row = m0 + wm * WTM + i * 32 + l32;
+    if (row >= p.M) continue;
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int col = n0 + wn * WTN + j * 32 + out_row<Elem::BF16, 32>(half, r);
+        if (col >= p.N) continue;
+        float *dst = p.C + (int64_t)col * p.ldc + row;
+        float o = p.alpha * (big[i][j][r] + small[i][j][r]);
+        if (p.beta != 0.f) o = __builtin_fmaf(p.beta, *dst, o);
+        *dst = o;
+      }
+    }
+  }
+}
+
+template <int BM, int BN, int WM, int WN, int SKB>
+hcblasStatus_t launch_split3(hipStream_t stream, Split3Params p) {
+  p.tiles_m = (int)cdiv(p.M, BM);
+  p.tiles_n = (int)cdiv(p.N, BN);
+  gemm_split3_kernel<BM, BN, WM, WN, SKB>
+      <<<p.tiles_m * p.tiles_n, 64 * WM * WN, 0, stream>>>(p);
+  return launch_status();
 }
 
 // Smallest e with |x| < 2^e (x finite, nonzero): frexp gives x = f*2^e,
@@ -611,10 +839,12 @@ hcblasStatus_t gemm_split_bf16(hcblasHandle_t h, const Canon &c, float alpha,
   if (alpha == 0.f || c.K == 0)
     return gemm_real<Elem::F32, float>(h, c, alpha, beta, C, ldc, cOff,
                                        strideC, Cptrs, batch);
-  constexpr int BK = 64;
-  const int64_t Kp = cdiv(c.K, BK) * BK;
-  const size_t a_bytes = sizeof(uint16_t) * 3 * (size_t)c.M * Kp;
-  const size_t b_bytes = sizeof(uint16_t) * 3 * (size_t)c.N * Kp;
+  // One K block per step (two blocks per step need 256 VGPRs and spill at
+  // 8 waves); K is padded to a whole number of blocks.
+  constexpr int SKB = 1;
+  const int nkb = (int)(cdiv(c.K, kS3K * SKB) * SKB);
+  const size_t a_bytes = sizeof(uint16_t) * kS3Row * (size_t)nkb * c.M;
+  const size_t b_bytes = sizeof(uint16_t) * kS3Row * (size_t)nkb * c.N;
   if (ensure_workspace(h->split_ws, a_bytes + b_bytes) != hipSuccess)
     return HCBLAS_STATUS_ALLOC_FAILED;
   uint16_t *Ap = static_cast<uint16_t *>(h->split_ws.ptr);
@@ -627,39 +857,31 @@ hcblasStatus_t gemm_split_bf16(hcblasHandle_t h, const Canon &c, float alpha,
     const float *A = static_cast<const float *>(bv.A[z]) + c.aOff;
     const float *B = static_cast<const float *>(bv.B[z]) + c.bOff;
     float *Cz = static_cast<float *>(bv.C[z]) + cOff;
-    dim3 ga((unsigned)cdiv(Kp, 32), (unsigned)cdiv(c.M, 32));
-    split_bf16_kernel<<<ga, 256, 0, h->stream>>>(c.M, c.K, A, c.lda, c.a_k, Ap,
-                                                 Kp, false);
-    dim3 gb((unsigned)cdiv(Kp, 32), (unsigned)cdiv(c.N, 32));
-    split_bf16_kernel<<<gb, 256, 0, h->stream>>>(c.N, c.K, B, c.ldb, c.b_k, Bp,
-                                                 Kp, true);
+    const unsigned kt = (unsigned)cdiv(nkb * kS3K, 32);
+    dim3 ga(kt, (unsigned)cdiv(c.M, 64));
+    split3_pack_kernel<<<ga, 256, 0, h->stream>>>(c.M, c.K, A, c.lda, c.a_k,
+                                                  Ap, nkb);
+    dim3 gb(kt, (unsigned)cdiv(c.N, 64));
+    split3_pack_kernel<<<gb, 256, 0, h->stream>>>(c.N, c.K, B, c.ldb, c.b_k,
+                                                  Bp, nkb);
     st = launch_status();
     if (st != HCBLAS_STATUS_SUCCESS) return st;
-    PlaneParams p{};
+    Split3Params p{};
     p.M = c.M;
     p.N = c.N;
-    p.Kp = Kp;
-    p.S = 3;
+    p.nkb = nkb;
     p.Ap = Ap;
     p.Bp = Bp;
-    // smallest group first: hi*lo + mid*mid + lo*hi, hi*mid + mid*hi, hi*hi
-    p.ngroups = 3;
-    p.a0[0] = 0, p.b0[0] = 0, p.np[0] = 3;
-    p.a0[1] = 0, p.b0[1] = 1, p.np[1] = 2;
-    p.a0[2] = 0, p.b0[2] = 2, p.np[2] = 1;
     p.C = Cz;
     p.ldc = ldc;
     p.alpha = alpha;
     p.beta = beta;
-    // 256x256 tiles, 8 waves (4 x 2, 64x128 each) on 32x32 BF16 MFMA blocks
-    // (42.5% of BF16 peak at 8192^3); smaller problems that cannot fill the
-    // CUs take 128x128.
-    if (cdiv(c.M, 256) * cdiv(c.N, 256) >= h->num_cus)
-      st = launch_planes<Elem::BF16, float, 256, 256, BK, 4, 2, 32>(h->stream,
-                                                                    p);
+    // 256x128 tiles, 8 waves of 64x64 (two accumulators each); problems
+    // that cannot fill the CUs take 128x128 with 4 waves.
+    if (cdiv(c.M, 256) * cdiv(c.N, 128) >= h->num_cus)
+      st = launch_split3<256, 128, 4, 2, SKB>(h->stream, p);
     else
-      st = launch_planes<Elem::BF16, float, 128, 128, BK, 2, 2, 32>(h->stream,
-                                                                    p);
+      st = launch_split3<128, 128, 2, 2, SKB>(h->stream, p);
     if (st != HCBLAS_STATUS_SUCCESS) return st;
   }
   return HCBLAS_STATUS_SUCCESS;
