@@ -26,9 +26,8 @@
 // integers with |A'|, |B'| <= 2^b. For pairwise coprime moduli m_1..m_n
 // (all <= 256, product P) the symmetric residues of A' and B' are INT8, one
 // INT8 GEMM per modulus gives A'B' mod m_t exactly (INT32 while
-// K * 128 * 128 < 2^31), and Garner's mixed-radix reconstruction with
-// symmetric digits recovers A'B' exactly whenever K * 2^(2b) < P/2, which
-// fixes b. n = 16 moduli give b = 55 bits per operand at K = 8192 with 16
+// K * 128 * 128 < 2^31), and the explicit CRT recovers A'B' exactly whenever
+// K * 2^(2b) < P/2, which fixes b. n = 16 moduli give b = 55 bits per operand at K = 8192 with 16
 // INT8 products, against 36 for eight slices of the scheme above.
 #include <climits>
 #include <cmath>
@@ -484,85 +483,126 @@ __device__ __forceinline__ constexpr int modulus() {
   return m[T];
 }
 
-// s mod M for 0 <= s < 2^30 without an integer division: float(s) is off by
-// at most 32 and 1/M by a relative 2^-24, so the float quotient is within 1
-// of s/M and one correction each way suffices.
+// Symmetric residue of an integer s (|s| < 2^30) modulo M, as an int whose
+// low byte is the int8 residue: in [-(M-1)/2, (M-1)/2] for odd M, and for
+// M = 256 s itself (its low byte is s mod 256 in two's complement). The first
+// float quotient is within 2 of s/M (float(s) is off by at most 64, 1/M and
+// the product by a relative 2^-24 each), so |r| < 2.5 M; the second pass on
+// that small exact value is exact.
 template <int M>
-__device__ __forceinline__ int mod_small(int s) {
-  constexpr float inv = 1.0f / (float)M;
-  const int q = (int)__builtin_floorf((float)s * inv);
-  int r = s - q * M;
-  r += r < 0 ? M : 0;
-  r -= r >= M ? M : 0;
-  return r;
+__device__ __forceinline__ int sym_residue(int s) {
+  if constexpr (M == 256) {
+    return s;
+  } else {
+    constexpr float inv = 1.0f / (float)M;
+    const int q = (int)__builtin_rintf((float)s * inv);
+    const float rf = (float)(s - q * M);
+    return (int)__builtin_fmaf(-__builtin_rintf(rf * inv), (float)M, rf);
+  }
 }
 
-// Symmetric residue of v = sign * (l2 2^42 + l1 2^21 + l0) (limbs < 2^21)
-// modulo M, as the byte of an int8 in [-M/2, (M-1)/2].
-template <int M>
-__device__ __forceinline__ uint32_t residue_byte(int l2, int l1, int l0,
-                                                 bool neg) {
-  constexpr int c21 = (int)((1ll << 21) % M);
-  constexpr int c42 = (int)((1ll << 42) % M);
-  int q = mod_small<M>(l2 * c42 + l1 * c21 + l0);  // < 2^30
-  if (neg && q != 0) q = M - q;
-  if (q > (M - 1) / 2) q -= M;
-  return (uint32_t)(q & 0xff);
+// Low bytes of four ints packed into one dword.
+__device__ __forceinline__ uint32_t pack_bytes(int b0, int b1, int b2, int b3) {
+  const uint32_t lo = __builtin_amdgcn_perm((uint32_t)b1, (uint32_t)b0,
+                                            0x0c0c0400u);
+  const uint32_t hi = __builtin_amdgcn_perm((uint32_t)b3, (uint32_t)b2,
+                                            0x0c0c0400u);
+  return __builtin_amdgcn_perm(hi, lo, 0x05040100u);
 }
 
-struct Limbs {
-  int l2[4], l1[4], l0[4];
-  bool neg[4];
+// Signed limbs of the integers v_j = l2 2^42 + l1 2^21 + l0 (|limb| < 2^21,
+// all limbs of v_j share its sign; WIDE = false: |v_j| < 2^30 and only l0 is
+// used).
+template <bool WIDE>
+struct Limbs16 {
+  int l2[WIDE ? 16 : 1], l1[WIDE ? 16 : 1], l0[16];
 };
 
-// Four consecutive k of one row: one packed dword per modulus plane.
-template <int NMOD, int T = 0>
-__device__ __forceinline__ void write_residues4(const Limbs &L, int8_t *row,
-                                                int64_t Kp) {
+// Residue planes of the 16 integers of one row segment: one 16-byte store
+// per modulus plane (plane t of the row at row + t * Kp).
+template <int NMOD, bool WIDE, int T = 0>
+__device__ __forceinline__ void write_residues16(const Limbs16<WIDE> &L,
+                                                 int8_t *row, int64_t Kp) {
   if constexpr (T < NMOD) {
     constexpr int M = modulus<T>();
-    uint32_t w = 0;
+    constexpr int c21 = (int)((1ll << 21) % M);
+    constexpr int c42 = (int)((1ll << 42) % M);
+    int q[16];
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
-      w |= residue_byte<M>(L.l2[j], L.l1[j], L.l0[j], L.neg[j]) << (8 * j);
-    *reinterpret_cast<uint32_t *>(row + (int64_t)T * Kp) = w;
-    write_residues4<NMOD, T + 1>(L, row, Kp);
+    for (int j = 0; j < 16; ++j) {
+      int sv = L.l0[j];
+      if constexpr (WIDE) {
+        if constexpr (M != 256) sv += L.l2[j] * c42 + L.l1[j] * c21;
+      }
+      q[j] = sym_residue<M>(sv);
+    }
+    vec4u w;
+    w.x = pack_bytes(q[0], q[1], q[2], q[3]);
+    w.y = pack_bytes(q[4], q[5], q[6], q[7]);
+    w.z = pack_bytes(q[8], q[9], q[10], q[11]);
+    w.w = pack_bytes(q[12], q[13], q[14], q[15]);
+    *reinterpret_cast<vec4u *>(row + (int64_t)T * Kp) = w;
+    write_residues16<NMOD, WIDE, T + 1>(L, row, Kp);
   }
 }
 
-// Residue planes P[r][t][0..Kp) of rint(X(r, k) * 2^(bits - e_r)). A
-// workgroup stages a 32 x 32 tile; each thread converts 4 consecutive k of
-// one row and writes one dword per plane (Kp is a multiple of 32).
-template <int NMOD>
+// Residue planes P[r][t][0..Kp) of rint(X(r, k) * 2^(bits - e_r)) (Kp a
+// multiple of 64; zeros past K). A workgroup stages a 64 x 64 tile through
+// LDS, reading along the source's contiguous dimension; each thread then
+// converts 16 consecutive k of one row and writes one 16-byte vector per
+// plane. FP64 input: |v| <= 2^62 in three 21-bit limbs; FP32 input: bits
+// <= 30, so |v| <= 2^30 is one limb.
+template <typename T, int NMOD>
 __global__ __launch_bounds__(256) void crt_residue_kernel(
-    int R, int K, const double *X, int64_t ld, bool kcont, const int *exps,
-    int bits, int8_t *P, int64_t Kp) {
-  __shared__ TileStage<double> st;
-  const int r0 = blockIdx.y * 32, k0 = blockIdx.x * 32;
-  st.load(X, ld, kcont, R, K, r0, k0);
-  const int i = threadIdx.x >> 3, kq = (threadIdx.x & 7) * 4;
-  const int r = r0 + i, k = k0 + kq;
+    int R, int K, const T *__restrict__ X, int64_t ld, bool kcont,
+    const int *__restrict__ exps, int bits, int8_t *__restrict__ P,
+    int64_t Kp) {
+  constexpr bool WIDE = sizeof(T) == 8;
+  __shared__ T st[64][65];
+  const int r0 = blockIdx.y * 64, k0 = blockIdx.x * 64;
+  const int tid = threadIdx.x;
+  if (kcont) {
+#pragma unroll 4
+    for (int p = 0; p < 16; ++p) {
+      const int idx = p * 256 + tid, i = idx >> 6, kk = idx & 63;
+      const int r = r0 + i, k = k0 + kk;
+      st[i][kk] = (r < R && k < K) ? X[(int64_t)r * ld + k] : T(0);
+    }
+  } else {
+#pragma unroll 4
+    for (int p = 0; p < 16; ++p) {
+      const int idx = p * 256 + tid, kk = idx >> 6, i = idx & 63;
+      const int r = r0 + i, k = k0 + kk;
+      st[i][kk] = (r < R && k < K) ? X[(int64_t)k * ld + r] : T(0);
+    }
+  }
+  __syncthreads();
+  const int row = tid >> 2, kq = (tid & 3) * 16;
+  const int r = r0 + row;
   if (r >= R) return;
   const int e = exps[r];
-  Limbs L;
+  Limbs16<WIDE> L;
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const double v = e == INT_MIN
-                         ? 0.0
-                         : __builtin_rint(
-                               __builtin_ldexp(st.v[i][kq + j], bits - e));
-    // |v| <= 2^62 is an integer; split it exactly into three 21-bit limbs.
-    const double a = __builtin_fabs(v);
-    const double a2 = __builtin_floor(a * 0x1p-42);
-    const double rem = a - a2 * 0x1p42;
-    const double a1 = __builtin_floor(rem * 0x1p-21);
-    const double a0 = rem - a1 * 0x1p21;
-    L.l2[j] = (int)a2;
-    L.l1[j] = (int)a1;
-    L.l0[j] = (int)a0;
-    L.neg[j] = v < 0.0;
+  for (int j = 0; j < 16; ++j) {
+    const double v =
+        e == INT_MIN
+            ? 0.0
+            : __builtin_rint(__builtin_ldexp((double)st[row][kq + j], bits - e));
+    if constexpr (WIDE) {
+      const double a = __builtin_fabs(v);
+      const double a2 = __builtin_floor(a * 0x1p-42);
+      const double rem = a - a2 * 0x1p42;
+      const double a1 = __builtin_floor(rem * 0x1p-21);
+      const double a0 = rem - a1 * 0x1p21;
+      const int sg = v < 0.0 ? -1 : 1;
+      L.l2[j] = sg * (int)a2;
+      L.l1[j] = sg * (int)a1;
+      L.l0[j] = sg * (int)a0;
+    } else {
+      L.l0[j] = (int)v;
+    }
   }
-  write_residues4<NMOD>(L, P + (int64_t)r * NMOD * Kp + k, Kp);
+  write_residues16<NMOD, WIDE>(L, P + (int64_t)r * NMOD * Kp + k0 + kq, Kp);
 }
 
 struct CrtParams {
@@ -606,160 +646,161 @@ __global__ __launch_bounds__(64 * WM * WN) void crt_gemm_kernel(CrtParams p) {
       });
 }
 
-// Reconstruction. The moduli are taken in groups of three (the last group
-// may hold one or two); a group's residues combine exactly, in 32-bit
-// integers, into one residue modulo the group product SM_g < 2^24 (Garner
-// within the group). Garner's algorithm with symmetric digits then runs over
-// the super-moduli SM_g in FP64 arithmetic, where every intermediate is an
-// integer below 2^50 and so exact, and the mixed-radix value is evaluated by
-// Horner's rule in double-double.
-constexpr long long inv_mod64(long long a, long long m) {
-  long long t = 0, nt = 1, r = m, nr = a % m;
-  while (nr != 0) {
-    const long long q = r / nr;
-    const long long t2 = t - q * nt;
-    t = nt;
-    nt = t2;
-    const long long r2 = r - q * nr;
-    r = nr;
-    nr = r2;
-  }
-  return t < 0 ? t + m : t;
-}
-template <int NMOD, int G>
-__device__ __forceinline__ constexpr long long super_modulus() {
-  constexpr int i0 = 3 * G;
-  long long p = modulus<i0>();
-  if constexpr (i0 + 1 < NMOD) p *= modulus<i0 + 1>();
-  if constexpr (i0 + 2 < NMOD) p *= modulus<i0 + 2>();
-  return p;
-}
-
-// Residues u of the moduli of group G -> residue modulo SM_G, in [0, SM_G).
-template <int NMOD, int G>
-__device__ __forceinline__ int group_residue(const int *u) {
-  constexpr int i0 = 3 * G;
-  constexpr int m0 = modulus<i0>();
-  if constexpr (i0 + 1 >= NMOD) {
-    return u[i0];
-  } else {
-    constexpr int m1 = modulus<i0 + 1>();
-    constexpr int inv01 = inv_mod(m0 % m1, m1);
-    // u1 - u0 + 2 m1 > 0 and < 2^10, times inv01 < 2^18
-    const int d1 = mod_small<m1>((u[i0 + 1] - u[i0] + 2 * m1) * inv01);
-    const int part = u[i0] + m0 * d1;  // residue modulo m0 m1
-    if constexpr (i0 + 2 >= NMOD) {
-      return part;
-    } else {
-      constexpr int m2 = modulus<i0 + 2>();
-      constexpr int inv2 = inv_mod((m0 * m1) % m2, m2);
-      constexpr int c = m2 * ((65536 + m2 - 1) / m2);  // >= m0 m1
-      const int d2 =
-          mod_small<m2>(mod_small<m2>(u[i0 + 2] - part + c) * inv2);
-      return part + m0 * m1 * d2;
+// Reconstruction by the explicit CRT. With P = prod m_t, M_t = P / m_t and
+// w_t = M_t * (M_t^-1 mod m_t) (< P), the integer C with C = u_t (mod m_t)
+// and |C| < P/2 is C = S - q P, S = sum_t u_t w_t, q = rint(S / P) (S < 20 *
+// 256 * P, so q < 2^13). Every w_t and P are split into 32-bit parts
+// (x = sum_j x_j 2^(32 j)); then
+//   S_j = sum_t u_t w_tj < 2^46   and   D_j = S_j - q P_j   (|D_j| < 2^46)
+// are exact in FP64, q comes from sum_j S_j (2^(32 j) / P) (absolute error
+// below 2^-38, while |C| / P <= 0.4965 by the choice of the operand bits),
+// and C = sum_j D_j 2^(32 j) is summed by Horner's rule in double-double, so
+// the returned double is C correctly rounded except within 2^-105 of a
+// rounding boundary, and exactly representable values come out exact.
+struct CrtBig {
+  uint32_t d[6];  // little-endian 32-bit limbs (P < 2^160 for 20 moduli)
+};
+constexpr CrtBig crt_product(int n) {
+  CrtBig p{};
+  p.d[0] = 1;
+  for (int t = 0; t < n; ++t) {
+    uint64_t carry = 0;
+    for (int i = 0; i < 6; ++i) {
+      const uint64_t x = (uint64_t)p.d[i] * (uint64_t)kModuli[t] + carry;
+      p.d[i] = (uint32_t)x;
+      carry = x >> 32;
     }
   }
+  return p;
 }
-
-// y mod SM for an integer-valued 0 <= y < 2^53 (rcp = 1/SM).
-__device__ __forceinline__ double dmod(double y, double SM, double rcp) {
-  const double q = __builtin_floor(y * rcp);
-  double r = __builtin_fma(-q, SM, y);  // exact: an integer of |r| < 2 SM
-  r += r < 0.0 ? SM : 0.0;
-  r -= r >= SM ? SM : 0.0;
-  return r;
-}
-
-// Garner digit T over the super-moduli (symmetric digits V[0..T)).
-template <int NMOD, int T, int S = 0>
-__device__ __forceinline__ double garner_super_step(double x, const double *V) {
-  if constexpr (S < T) {
-    constexpr long long SMt = super_modulus<NMOD, T>();
-    constexpr long long SMs = super_modulus<NMOD, S>();
-    constexpr double inv = (double)inv_mod64(SMs % SMt, SMt);
-    // a multiple of SM_t above 2^23 > |V[S]|
-    constexpr double c = (double)(SMt * (((1ll << 23) + SMt - 1) / SMt));
-    const double y = (x - V[S] + c) * inv;  // integer < 2^50: exact
-    x = dmod(y, (double)SMt, 1.0 / (double)SMt);
-    return garner_super_step<NMOD, T, S + 1>(x, V);
-  } else {
-    return x;
+constexpr CrtBig big_div_small(CrtBig a, uint32_t m) {
+  uint64_t rem = 0;
+  for (int i = 5; i >= 0; --i) {
+    const uint64_t cur = (rem << 32) | a.d[i];
+    a.d[i] = (uint32_t)(cur / m);
+    rem = cur % m;
   }
+  return a;
 }
-template <int NMOD, int NG, int T = 0>
-__device__ __forceinline__ void garner_super(const int *u, double *V) {
-  if constexpr (T < NG) {
-    constexpr long long SM = super_modulus<NMOD, T>();
-    const double x =
-        garner_super_step<NMOD, T>((double)group_residue<NMOD, T>(u), V);
-    V[T] = x > (double)((SM - 1) / 2) ? x - (double)SM : x;
-    garner_super<NMOD, NG, T + 1>(u, V);
+constexpr uint32_t big_mod_small(const CrtBig &a, uint32_t m) {
+  uint64_t rem = 0;
+  for (int i = 5; i >= 0; --i) rem = ((rem << 32) | a.d[i]) % m;
+  return (uint32_t)rem;
+}
+constexpr CrtBig big_mul_small(CrtBig a, uint32_t m) {
+  uint64_t carry = 0;
+  for (int i = 0; i < 6; ++i) {
+    const uint64_t x = (uint64_t)a.d[i] * m + carry;
+    a.d[i] = (uint32_t)x;
+    carry = x >> 32;
   }
+  return a;
+}
+constexpr int crt_parts(int n) {
+  const CrtBig p = crt_product(n);
+  int top = 5;
+  while (top > 0 && p.d[top] == 0) --top;
+  return top + 1;
+}
+template <int NMOD>
+struct CrtRecon {
+  static constexpr int NP = crt_parts(NMOD);
+  double w[NMOD][NP];  // 32-bit parts of w_t
+  double p[NP];        // 32-bit parts of P
+  double f[NP];        // 2^(32 j) / P, rounded
+};
+template <int NMOD>
+constexpr CrtRecon<NMOD> crt_recon() {
+  CrtRecon<NMOD> c{};
+  constexpr int NP = CrtRecon<NMOD>::NP;
+  const CrtBig P = crt_product(NMOD);
+  double pd = 0.0;
+  for (int i = 5; i >= 0; --i) pd = pd * 4294967296.0 + (double)P.d[i];
+  double scale = 1.0;
+  for (int j = 0; j < NP; ++j) {
+    c.p[j] = (double)P.d[j];
+    c.f[j] = scale / pd;
+    scale *= 4294967296.0;
+  }
+  for (int t = 0; t < NMOD; ++t) {
+    const uint32_t m = (uint32_t)kModuli[t];
+    const CrtBig Mt = big_div_small(P, m);
+    const uint32_t y = (uint32_t)inv_mod((int)big_mod_small(Mt, m), (int)m);
+    const CrtBig w = big_mul_small(Mt, y);
+    for (int j = 0; j < NP; ++j) c.w[t][j] = (double)w.d[j];
+  }
+  return c;
 }
 
-// Mixed-radix value sum_t V_t * prod_{s<t} SM_s by Horner's rule in
-// double-double (about 106 bits; the exact value has up to ~126), so the
-// returned double is the correctly rounded value except within 2^-105 of a
-// rounding boundary, and exactly representable results come out exact.
-template <int NMOD, int NG, int T>
-__device__ __forceinline__ void horner_dd(const double *V, double &hi,
-                                          double &lo) {
-  if constexpr (T == NG - 1) {
-    hi = V[T];
-    lo = 0.0;
-  } else {
-#pragma clang fp contract(off)
-    horner_dd<NMOD, NG, T + 1>(V, hi, lo);
-    const double m = (double)super_modulus<NMOD, T>();
-    const double p = hi * m;
-    const double e = __builtin_fma(hi, m, -p);  // hi*m = p + e exactly
-    const double vt = V[T];
-    const double s = p + vt;                    // two-sum: p + vt = s + t
-    const double bb = s - p;
-    const double t = (p - (s - bb)) + (vt - bb);
-    const double l = __builtin_fma(lo, m, e + t);
-    hi = s + l;                                 // renormalise
-    lo = l - (hi - s);
-  }
-}
-
-// One output element from its residues.
+// C from its residues u_t in [0, m_t).
 template <int NMOD>
 __device__ __forceinline__ double crt_value(const int *u) {
-  constexpr int NG = (NMOD + 2) / 3;
-  double V[NG];
-  garner_super<NMOD, NG>(u, V);
-  double hi, lo;
-  horner_dd<NMOD, NG, 0>(V, hi, lo);
+#pragma clang fp contract(off)
+  constexpr CrtRecon<NMOD> c = crt_recon<NMOD>();
+  constexpr int NP = CrtRecon<NMOD>::NP;
+  double S[NP];
+#pragma unroll
+  for (int j = 0; j < NP; ++j) S[j] = 0.0;
+#pragma unroll
+  for (int t = 0; t < NMOD; ++t) {
+    const double ut = (double)u[t];
+#pragma unroll
+    for (int j = 0; j < NP; ++j) S[j] = __builtin_fma(ut, c.w[t][j], S[j]);
+  }
+  double qf = 0.0;
+#pragma unroll
+  for (int j = 0; j < NP; ++j) qf = __builtin_fma(S[j], c.f[j], qf);
+  const double q = __builtin_rint(qf);
+  double hi = __builtin_fma(-q, c.p[NP - 1], S[NP - 1]), lo = 0.0;
+#pragma unroll
+  for (int j = NP - 2; j >= 0; --j) {
+    const double dj = __builtin_fma(-q, c.p[j], S[j]);
+    hi *= 4294967296.0;
+    lo *= 4294967296.0;
+    const double sm = hi + dj;  // two-sum: hi + dj = sm + e
+    const double bb = sm - hi;
+    const double e = (hi - (sm - bb)) + (dj - bb);
+    const double l = lo + e;
+    hi = sm + l;  // renormalise
+    lo = l - (hi - sm);
+  }
   return hi + lo;
 }
 
+template <typename OutT>
 __device__ __forceinline__ void crt_store(double x, int re, int ce, int shift,
                                           double alpha, double beta,
-                                          double *dst) {
+                                          OutT *dst) {
   const double val = (re == INT_MIN || ce == INT_MIN)
                          ? 0.0
                          : __builtin_ldexp(x, re + ce - shift);
-  double o = alpha * val;
-  if (beta != 0.0) o = __builtin_fma(beta, *dst, o);
-  *dst = o;
+  if constexpr (sizeof(OutT) == 8) {
+    double o = alpha * val;
+    if (beta != 0.0) o = __builtin_fma(beta, (double)*dst, o);
+    *dst = (OutT)o;
+  } else {
+    // FP32 output: alpha * AB rounded once to FP32, then beta C in FP32.
+    float o = (float)(alpha * val);
+    if (beta != 0.0) o = __builtin_fmaf((float)beta, (float)*dst, o);
+    *dst = (OutT)o;
+  }
 }
 
 // C(m, n) = alpha * 2^(e_m + f_n - shift) * CRT(R[.][n][m]) + beta * C(m, n).
 // VEC4 (M % 4 == 0): a thread takes 4 consecutive m of one column and reads
 // one dword per residue plane.
-template <int NMOD, bool VEC4>
+template <int NMOD, typename OutT, bool VEC4>
 __global__ __launch_bounds__(256) void crt_combine_kernel(
-    int M, int N, const uint8_t *R, const int *ea, const int *eb, int shift,
-    double alpha, double beta, double *C, int64_t ldc) {
-#pragma clang fp contract(off)
+    int M, int N, const uint8_t *__restrict__ R, const int *__restrict__ ea,
+    const int *__restrict__ eb, int shift, double alpha, double beta,
+    OutT *C, int64_t ldc) {
   const int64_t MN = (int64_t)M * N;
   constexpr int W = VEC4 ? 4 : 1;
   const int64_t idx = ((int64_t)blockIdx.x * 256 + threadIdx.x) * W;
   if (idx >= MN) return;
   const int m = (int)(idx % M), n = (int)(idx / M);
   const int ce = eb[n];
-  double *dst = C + (int64_t)n * ldc + m;
+  OutT *dst = C + (int64_t)n * ldc + m;
   if constexpr (VEC4) {
     uint32_t w[NMOD];
 #pragma unroll
@@ -781,25 +822,27 @@ __global__ __launch_bounds__(256) void crt_combine_kernel(
   }
 }
 
-template <int NMOD>
-void launch_crt_residues(hipStream_t s, int R, int K, const double *X,
-                         int64_t ld, bool kcont, const int *exps, int bits,
-                         int8_t *P, int64_t Kp) {
-  dim3 g((unsigned)cdiv(Kp, 32), (unsigned)cdiv(R, 32));
-  crt_residue_kernel<NMOD><<<g, 256, 0, s>>>(R, K, X, ld, kcont, exps, bits,
-                                            P, Kp);
+template <int NMOD, typename T>
+void launch_crt_residues(hipStream_t s, int R, int K, const T *X, int64_t ld,
+                         bool kcont, const int *exps, int bits, int8_t *P,
+                         int64_t Kp) {
+  dim3 g((unsigned)cdiv(Kp, 64), (unsigned)cdiv(R, 64));
+  crt_residue_kernel<T, NMOD><<<g, 256, 0, s>>>(R, K, X, ld, kcont, exps,
+                                               bits, P, Kp);
 }
-template <int NMOD>
+template <int NMOD, typename OutT>
 void launch_crt_combine(hipStream_t s, int M, int N, const uint8_t *R,
                         const int *ea, const int *eb, int shift, double alpha,
-                        double beta, double *C, int64_t ldc) {
+                        double beta, OutT *C, int64_t ldc) {
   const int64_t MN = (int64_t)M * N;
   if (M % 4 == 0)
-    crt_combine_kernel<NMOD, true><<<(unsigned)cdiv(MN / 4, 256), 256, 0, s>>>(
-        M, N, R, ea, eb, shift, alpha, beta, C, ldc);
+    crt_combine_kernel<NMOD, OutT, true>
+        <<<(unsigned)cdiv(MN / 4, 256), 256, 0, s>>>(M, N, R, ea, eb, shift,
+                                                     alpha, beta, C, ldc);
   else
-    crt_combine_kernel<NMOD, false><<<(unsigned)cdiv(MN, 256), 256, 0, s>>>(
-        M, N, R, ea, eb, shift, alpha, beta, C, ldc);
+    crt_combine_kernel<NMOD, OutT, false>
+        <<<(unsigned)cdiv(MN, 256), 256, 0, s>>>(M, N, R, ea, eb, shift,
+                                                 alpha, beta, C, ldc);
 }
 
 // Runtime modulus count -> template instance (8..20).
@@ -973,7 +1016,7 @@ hcblasStatus_t gemm_ozaki_int8(hcblasHandle_t h, const Canon &c, double alpha,
       p.ldc = ldc;
       p.alpha = alpha;
       p.beta = k0 == 0 ? beta : 1.0;
-      st = launch_planes<Elem::I8, double, 128, 128, BK>(h->stream, p);
+      st = launch_planes<Elem::I8, double, 128, 128, BK, 2, 2, 32>(h->stream, p);
       if (st != HCBLAS_STATUS_SUCCESS) return st;
     }
   }
@@ -982,11 +1025,13 @@ hcblasStatus_t gemm_ozaki_int8(hcblasHandle_t h, const Canon &c, double alpha,
 
 // ---------------------------------------------------------------------------
 // Bits per operand for n moduli and a K chunk of kc: the largest b with
-// kc * 2^(2b) < P / 2 (P = product of the moduli), at most 62.
+// kc * 2^(2b) < P / 2^1.01 (P = product of the moduli), at most 62.
 static int crt_bits(int nmod, int64_t kc) {
   double log2p = 0.0;
   for (int t = 0; t < nmod; ++t) log2p += std::log2((double)kModuli[t]);
-  const double room = log2p - 1.0 - std::log2((double)kc) - 1e-6;
+  // 0.01 bit of margin keeps |C| / P <= 0.4965, which the reconstruction's
+  // rounding of S / P relies on.
+  const double room = log2p - 1.0 - std::log2((double)kc) - 0.01;
   int b = (int)std::floor(room / 2.0);
   return b > 62 ? 62 : b;
 }
@@ -1016,7 +1061,7 @@ hcblasStatus_t gemm_ozaki_crt(hcblasHandle_t h, const Canon &c, double alpha,
                                         strideC, Cptrs, batch);
   // 256x256 tiles, 8 waves on 16x16 INT8 MFMA blocks (38.5% of INT8 peak
   // at 8192^2 x 16 moduli, against 28.8% for 128x128).
-  constexpr int BM = 256, BN = 256, BK = 128, WM = 2, WN = 4, MI = 16;
+  constexpr int BM = 256, BN = 256, BK = 128, WM = 2, WN = 4, MI = 32;
   // INT32 exactness of one modulus product: kc * 128 * 128 < 2^31.
   constexpr int64_t kmax = 65536;
   const int64_t kchunk = std::min<int64_t>(cdiv(c.K, BK) * BK, kmax);
