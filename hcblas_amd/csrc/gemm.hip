@@ -436,6 +436,9 @@ hcblasStatus_t gemm_complex(hcblasHandle_t h, const Canon &c, T ar, T ai,
   // Pointer-array C: the kernel adds cOff in real elements. The embedded
   // real problem follows the handle's math mode (split CGEMM, Ozaki ZGEMM).
   if constexpr (E == Elem::F32) {
+    if (h->math_mode & HCBLAS_OZAKI_CRT_F32_MATH)
+      return gemm_ozaki_crt_f32(h, r, 1.f, (float)br, static_cast<float *>(C),
+                                2 * ldc, 2 * cOff, 2 * strideC, Cptrs, batch);
     if (h->math_mode & HCBLAS_SPLIT_BF16_MATH)
       return gemm_split_bf16(h, r, 1.f, (float)br, static_cast<float *>(C),
                              2 * ldc, 2 * cOff, 2 * strideC, Cptrs, batch);
@@ -539,6 +542,9 @@ hcblasStatus_t dispatch(hcblasHandle_t h, hcblasDatatype_t dt, const Canon &c,
     case HCBLAS_R_32F: {
       const float a = *static_cast<const float *>(alpha);
       const float b = *static_cast<const float *>(beta);
+      if (mode & HCBLAS_OZAKI_CRT_F32_MATH)
+        return gemm_ozaki_crt_f32(h, c, a, b, static_cast<float *>(C), ldc,
+                                  cOff, strideC, Cptrs, batch);
       if (mode & HCBLAS_SPLIT_BF16_MATH)
         return gemm_split_bf16(h, c, a, b, static_cast<float *>(C), ldc, cOff,
                                strideC, Cptrs, batch);
@@ -806,6 +812,16 @@ hcblasStatus_t hcblas_sgemm_split_bf16(
                      A, lda, aOffset, 0, nullptr, B, ldb, bOffset, 0, nullptr,
                      beta, C, ldc, cOffset, 0, nullptr, 1,
                      HCBLAS_SPLIT_BF16_MATH);
+}
+hcblasStatus_t hcblas_sgemm_ozaki_crt(
+    hcblasHandle_t handle, hcblasOrder_t order, hcblasOperation_t typeA,
+    hcblasOperation_t typeB, int M, int N, int K, const float *alpha,
+    const float *A, int64_t lda, const float *B, int64_t ldb, const float *beta,
+    float *C, int64_t ldc, int64_t aOffset, int64_t bOffset, int64_t cOffset) {
+  return gemm_common(handle, HCBLAS_R_32F, order, typeA, typeB, M, N, K, alpha,
+                     A, lda, aOffset, 0, nullptr, B, ldb, bOffset, 0, nullptr,
+                     beta, C, ldc, cOffset, 0, nullptr, 1,
+                     HCBLAS_OZAKI_CRT_F32_MATH);
 }
 hcblasStatus_t hcblas_dgemm_ozaki_int8(
     hcblasHandle_t handle, hcblasOrder_t order, hcblasOperation_t typeA,

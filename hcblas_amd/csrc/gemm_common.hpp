@@ -37,5 +37,9 @@ hcblasStatus_t gemm_ozaki_crt(hcblasHandle_t h, const Canon &c, double alpha,
                               double beta, double *C, int64_t ldc,
                               int64_t cOff, int64_t strideC,
                               void *const *Cptrs, int64_t batch);
+hcblasStatus_t gemm_ozaki_crt_f32(hcblasHandle_t h, const Canon &c,
+                                  float alpha, float beta, float *C,
+                                  int64_t ldc, int64_t cOff, int64_t strideC,
+                                  void *const *Cptrs, int64_t batch);
 
 }  // namespace hcb

@@ -194,10 +194,12 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_planes_kernel(
         OutT *dst = C + (int64_t)col * p.ldc + row;
         if constexpr (INT) {
           // An all-zero row / column has no exponent (INT_MIN) and a zero
-          // accumulator.
+          // accumulator; a row / column holding an Inf or NaN is marked
+          // INT_MAX and its outputs are NaN.
           const int re = p.row_exp[row];
           const double v =
-              (re == INT_MIN || ce == INT_MIN)
+              (re == INT_MAX || ce == INT_MAX) ? __builtin_nan("")
+              : (re == INT_MIN || ce == INT_MIN)
                   ? 0.0
                   : __builtin_ldexp(facc[i][j][r], re + ce + p.exp_bias);
           double o = p.alpha * v;

@@ -77,6 +77,60 @@ struct TileStage {
   }
 };
 
+// 64 (r) x 64 (k) tile of an R x K operand view staged through LDS (row
+// stride 65 elements), read along the source's contiguous dimension. With
+// vec (the base pointer and ld are multiples of the 16-byte vector) every
+// lane reads whole 16-byte vectors; partial vectors at the edges and
+// unaligned operands are read element by element, with zeros outside the
+// view.
+template <typename T>
+struct Tile64 {
+  T v[64][65];
+  __device__ __forceinline__ void load(const T *__restrict__ X, int64_t ld,
+                                       bool kcont, bool vec, int R, int K,
+                                       int r0, int k0) {
+    constexpr int VE = 16 / (int)sizeof(T);  // elements per vector
+    constexpr int LPL = 64 / VE;             // lanes per 64-element line
+    constexpr int LPP = 256 / LPL;           // lines per pass
+    const int line0 = threadIdx.x / LPL, c = (threadIdx.x % LPL) * VE;
+#pragma unroll
+    for (int p = 0; p < 64 / LPP; ++p) {
+      const int line = line0 + p * LPP;
+      const int r = kcont ? r0 + line : r0 + c;
+      const int k = kcont ? k0 + c : k0 + line;
+      const T *src = kcont ? X + (int64_t)r * ld + k : X + (int64_t)k * ld + r;
+      T t[VE];
+      const bool full = kcont ? (r < R && k + VE <= K) : (k < K && r + VE <= R);
+      if (vec && full) {
+        const vec4u w = *reinterpret_cast<const vec4u *>(src);
+        __builtin_memcpy(t, &w, 16);
+      } else {
+#pragma unroll
+        for (int e = 0; e < VE; ++e) {
+          const bool in = kcont ? (r < R && k + e < K) : (k < K && r + e < R);
+          t[e] = in ? src[e] : T(0);
+        }
+      }
+#pragma unroll
+      for (int e = 0; e < VE; ++e) {
+        if (kcont)
+          v[line][c + e] = t[e];
+        else
+          v[c + e][line] = t[e];
+      }
+    }
+    __syncthreads();
+  }
+};
+
+// True when 16-byte vector reads of an operand with leading dimension ld are
+// aligned (the base pointer and every line start).
+template <typename T>
+inline bool vec_aligned(const T *X, int64_t ld) {
+  return (reinterpret_cast<uintptr_t>(X) % 16 == 0) &&
+         ((ld * (int64_t)sizeof(T)) % 16 == 0);
+}
+
 // ---------------------------------------------------------------------------
 // Fused BF16x3 split GEMM.
 //
@@ -328,9 +382,16 @@ hcblasStatus_t launch_split3(hipStream_t stream, Split3Params p) {
   return launch_status();
 }
 
-// Smallest e with |x| < 2^e (x finite, nonzero): frexp gives x = f*2^e,
-// 0.5 <= |f| < 1.
+// Row markers shared by the exponent / norm scans and their consumers:
+// kNoScale for a row without a nonzero (it contributes zeros), kNonFinite for
+// a row holding an Inf or NaN (every output it feeds becomes NaN).
+constexpr int kNoScale = INT_MIN;
+constexpr int kNonFinite = INT_MAX;
+
+// Smallest e with |x| < 2^e for finite nonzero x (frexp gives x = f*2^e,
+// 0.5 <= |f| < 1); kNonFinite for Inf / NaN.
 __device__ __forceinline__ int exp_bound(double x) {
+  if (!__builtin_isfinite(x)) return kNonFinite;
   int e;
   (void)frexp(x, &e);
   return e;
@@ -342,38 +403,90 @@ __global__ void fill_int_kernel(int *p, int n, int v) {
 }
 
 // Row exponents: exps[r] = max over nonzero x of ilogb(x) + 1, so that
-// |x| < 2^exps[r]; rows without a nonzero keep INT_MIN.
-// k-contiguous rows: a workgroup reduces 1024 k of one row.
-__global__ __launch_bounds__(256) void row_exp_kcont_kernel(
-    int R, int K, const double *X, int64_t ld, int *exps) {
-  const int r = blockIdx.x;
-  const int kb = blockIdx.y * 1024;
-  int e = INT_MIN;
-  for (int k = kb + threadIdx.x; k < K && k < kb + 1024; k += 256) {
-    const double x = X[(int64_t)r * ld + k];
+// |x| < 2^exps[r]; rows without a nonzero keep kNoScale, rows with an Inf or
+// NaN get kNonFinite (the largest int, so the max keeps it). A workgroup
+// scans a 64 x 64 tile; four threads share a row (16 k each).
+__global__ __launch_bounds__(256) void row_exp_kernel(
+    int R, int K, const double *__restrict__ X, int64_t ld, bool kcont,
+    bool vec, int *exps) {
+  __shared__ Tile64<double> st;
+  const int r0 = blockIdx.y * 64, k0 = blockIdx.x * 64;
+  st.load(X, ld, kcont, vec, R, K, r0, k0);
+  const int row = threadIdx.x >> 2, kq = (threadIdx.x & 3) * 16;
+  int e = kNoScale;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const double x = st.v[row][kq + j];
     if (x != 0.0) e = max(e, exp_bound(x));
   }
-  for (int off = 32; off >= 1; off >>= 1) e = max(e, __shfl_xor(e, off, 64));
-  __shared__ int we[4];
-  if ((threadIdx.x & 63) == 0) we[threadIdx.x >> 6] = e;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int m = max(max(we[0], we[1]), max(we[2], we[3]));
-    if (m != INT_MIN) atomicMax(&exps[r], m);
-  }
+  e = max(e, __shfl_xor(e, 1, 64));
+  e = max(e, __shfl_xor(e, 2, 64));
+  if ((threadIdx.x & 3) == 0 && r0 + row < R && e != kNoScale)
+    atomicMax(&exps[r0 + row], e);
 }
-// r-contiguous rows: thread per row over a k range of 256.
-__global__ __launch_bounds__(256) void row_exp_rcont_kernel(
-    int R, int K, const double *X, int64_t ld, int *exps) {
-  const int r = blockIdx.x * 256 + threadIdx.x;
-  if (r >= R) return;
-  const int kb = blockIdx.y * 256;
-  int e = INT_MIN;
-  for (int k = kb; k < K && k < kb + 256; ++k) {
-    const double x = X[(int64_t)k * ld + r];
-    if (x != 0.0) e = max(e, exp_bound(x));
+
+// Row sums of squares in FP64, in two deterministic stages: partial sums
+// part[b][r] over 64-wide k blocks b (a 64 x 64 tile per workgroup), then
+// row_norm_shift_kernel adds them in block order. FP32 input: squares are
+// exact in FP64 and cannot overflow.
+__global__ __launch_bounds__(256) void row_sumsq_kernel(
+    int R, int K, const float *__restrict__ X, int64_t ld, bool kcont,
+    bool vec, double *part) {
+  __shared__ Tile64<float> st;
+  const int r0 = blockIdx.y * 64, k0 = blockIdx.x * 64;
+  st.load(X, ld, kcont, vec, R, K, r0, k0);
+  const int row = threadIdx.x >> 2, kq = (threadIdx.x & 3) * 16;
+  double s = 0.0;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const double x = (double)st.v[row][kq + j];
+    s = __builtin_fma(x, x, s);
   }
-  if (e != INT_MIN) atomicMax(&exps[r], e);
+  s += __shfl_xor(s, 1, 64);
+  s += __shfl_xor(s, 2, 64);
+  if ((threadIdx.x & 3) == 0 && r0 + row < R)
+    part[(int64_t)blockIdx.x * R + r0 + row] = s;
+}
+// Per-row scale for the norm-bounded CRT operands: with s = ||x_r||_2 the
+// row is scaled by 2^sigma, sigma the largest integer with
+// 2^sigma s + sqrt(kc)/2 <= 2^beta (so ||rint(x_r 2^sigma)||_2 <= 2^beta);
+// stored as exps[r] = -sigma, the form the residue and reconstruction
+// kernels take with bits = 0 and shift = 0. Zero rows get kNoScale, rows
+// with a non-finite sum kNonFinite.
+__global__ __launch_bounds__(256) void row_norm_shift_kernel(
+    int R, int nblk, const double *part, int beta, int kc, int *exps) {
+  // One wave per row: lane l adds blocks l, l + 64, ... in order, then a
+  // fixed shuffle tree combines the lanes (the same sum on every run).
+  const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (r >= R) return;
+  double s = 0.0;
+  for (int b = lane; b < nblk; b += 64) s += part[(int64_t)b * R + r];
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane != 0) return;
+  if (s == 0.0) {
+    exps[r] = kNoScale;
+    return;
+  }
+  if (!__builtin_isfinite(s)) {
+    exps[r] = kNonFinite;
+    return;
+  }
+  // Slack of 2^-30 relative covers the rounding of the sum, the square root
+  // and the quotient.
+  const double room =
+      __builtin_ldexp(1.0, beta) - 0.5 * __builtin_sqrt((double)kc) - 1.0;
+  const double t = room / (__builtin_sqrt(s) * (1.0 + 0x1p-30));
+  int e;
+  (void)frexp(t, &e);  // t = f 2^e, 0.5 <= f < 1: floor(log2 t) = e - 1
+  exps[r] = -(e - 1);
+}
+
+void launch_row_exp(hipStream_t s, int R, int K, const double *X, int64_t ld,
+                    bool kcont, int *exps) {
+  row_exp_kernel<<<dim3((unsigned)cdiv(K, 64), (unsigned)cdiv(R, 64)), 256, 0,
+                   s>>>(R, K, X, ld, kcont, vec_aligned(X, ld), exps);
 }
 
 // INT8 slices into planes P[r][q][0..Kp) (q = t, or s-1-t when reverse).
@@ -388,7 +501,9 @@ __global__ __launch_bounds__(256) void ozaki_slice_kernel(
     const int r = r0 + i, k = k0 + tx;
     if (r >= R || k >= Kp) continue;
     const int e = exps[r];
-    double v = e == INT_MIN ? 0.0 : __builtin_ldexp(st.v[i][tx], 6 - e);
+    double v = (e == kNoScale || e == kNonFinite)
+                   ? 0.0
+                   : __builtin_ldexp(st.v[i][tx], 6 - e);
     int8_t *row = P + (int64_t)r * s * Kp + k;
     for (int t = 0; t < s; ++t) {
       const double a = __builtin_rint(v);
@@ -554,29 +669,14 @@ __device__ __forceinline__ void write_residues16(const Limbs16<WIDE> &L,
 // <= 30, so |v| <= 2^30 is one limb.
 template <typename T, int NMOD>
 __global__ __launch_bounds__(256) void crt_residue_kernel(
-    int R, int K, const T *__restrict__ X, int64_t ld, bool kcont,
+    int R, int K, const T *__restrict__ X, int64_t ld, bool kcont, bool vec,
     const int *__restrict__ exps, int bits, int8_t *__restrict__ P,
     int64_t Kp) {
   constexpr bool WIDE = sizeof(T) == 8;
-  __shared__ T st[64][65];
+  __shared__ Tile64<T> st;
   const int r0 = blockIdx.y * 64, k0 = blockIdx.x * 64;
   const int tid = threadIdx.x;
-  if (kcont) {
-#pragma unroll 4
-    for (int p = 0; p < 16; ++p) {
-      const int idx = p * 256 + tid, i = idx >> 6, kk = idx & 63;
-      const int r = r0 + i, k = k0 + kk;
-      st[i][kk] = (r < R && k < K) ? X[(int64_t)r * ld + k] : T(0);
-    }
-  } else {
-#pragma unroll 4
-    for (int p = 0; p < 16; ++p) {
-      const int idx = p * 256 + tid, kk = idx >> 6, i = idx & 63;
-      const int r = r0 + i, k = k0 + kk;
-      st[i][kk] = (r < R && k < K) ? X[(int64_t)k * ld + r] : T(0);
-    }
-  }
-  __syncthreads();
+  st.load(X, ld, kcont, vec, R, K, r0, k0);
   const int row = tid >> 2, kq = (tid & 3) * 16;
   const int r = r0 + row;
   if (r >= R) return;
@@ -585,9 +685,9 @@ __global__ __launch_bounds__(256) void crt_residue_kernel(
 #pragma unroll
   for (int j = 0; j < 16; ++j) {
     const double v =
-        e == INT_MIN
+        (e == kNoScale || e == kNonFinite)
             ? 0.0
-            : __builtin_rint(__builtin_ldexp((double)st[row][kq + j], bits - e));
+            : __builtin_rint(__builtin_ldexp((double)st.v[row][kq + j], bits - e));
     if constexpr (WIDE) {
       const double a = __builtin_fabs(v);
       const double a2 = __builtin_floor(a * 0x1p-42);
@@ -767,58 +867,76 @@ __device__ __forceinline__ double crt_value(const int *u) {
   return hi + lo;
 }
 
+// alpha * 2^(re + ce - shift) * x, or NaN for a row / column marked
+// non-finite, or 0 for an all-zero row / column.
+__device__ __forceinline__ double crt_scaled(double x, int re, int ce,
+                                             int shift, double alpha) {
+  if (re == kNonFinite || ce == kNonFinite) return __builtin_nan("");
+  if (re == kNoScale || ce == kNoScale) return 0.0;
+  return alpha * __builtin_ldexp(x, re + ce - shift);
+}
+// o = v + beta * c in the output precision (FP32: v rounded once, then the
+// beta term added in FP32).
 template <typename OutT>
-__device__ __forceinline__ void crt_store(double x, int re, int ce, int shift,
-                                          double alpha, double beta,
-                                          OutT *dst) {
-  const double val = (re == INT_MIN || ce == INT_MIN)
-                         ? 0.0
-                         : __builtin_ldexp(x, re + ce - shift);
+__device__ __forceinline__ OutT crt_out(double v, double beta, OutT c) {
   if constexpr (sizeof(OutT) == 8) {
-    double o = alpha * val;
-    if (beta != 0.0) o = __builtin_fma(beta, (double)*dst, o);
-    *dst = (OutT)o;
+    return beta != 0.0 ? __builtin_fma(beta, (double)c, v) : v;
   } else {
-    // FP32 output: alpha * AB rounded once to FP32, then beta C in FP32.
-    float o = (float)(alpha * val);
-    if (beta != 0.0) o = __builtin_fmaf((float)beta, (float)*dst, o);
-    *dst = (OutT)o;
+    const float o = (float)v;
+    return beta != 0.0 ? __builtin_fmaf((float)beta, (float)c, o) : o;
   }
 }
 
 // C(m, n) = alpha * 2^(e_m + f_n - shift) * CRT(R[.][n][m]) + beta * C(m, n).
-// VEC4 (M % 4 == 0): a thread takes 4 consecutive m of one column and reads
-// one dword per residue plane.
-template <int NMOD, typename OutT, bool VEC4>
+// W = 4 (M % 4 == 0): a thread takes 4 consecutive m of one column, reads one
+// dword per residue plane and, when C is vector aligned (vec_c), writes its
+// outputs with 16-byte stores. W = 1: one element. (Sixteen elements per
+// thread measured slower: the serial reconstructions and the register use
+// cost more parallelism than the wider loads gain.)
+template <int NMOD, typename OutT, int W>
 __global__ __launch_bounds__(256) void crt_combine_kernel(
     int M, int N, const uint8_t *__restrict__ R, const int *__restrict__ ea,
     const int *__restrict__ eb, int shift, double alpha, double beta,
-    OutT *C, int64_t ldc) {
+    OutT *C, int64_t ldc, bool vec_c) {
   const int64_t MN = (int64_t)M * N;
-  constexpr int W = VEC4 ? 4 : 1;
   const int64_t idx = ((int64_t)blockIdx.x * 256 + threadIdx.x) * W;
   if (idx >= MN) return;
   const int m = (int)(idx % M), n = (int)(idx / M);
   const int ce = eb[n];
   OutT *dst = C + (int64_t)n * ldc + m;
-  if constexpr (VEC4) {
+  if constexpr (W == 4) {
     uint32_t w[NMOD];
 #pragma unroll
     for (int t = 0; t < NMOD; ++t)
       w[t] = *reinterpret_cast<const uint32_t *>(R + t * MN + idx);
+    OutT o[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       int u[NMOD];
 #pragma unroll
       for (int t = 0; t < NMOD; ++t) u[t] = (w[t] >> (8 * j)) & 0xff;
-      crt_store(crt_value<NMOD>(u), ea[m + j], ce, shift, alpha, beta,
-                dst + j);
+      const double v = crt_scaled(crt_value<NMOD>(u), ea[m + j], ce, shift,
+                                  alpha);
+      o[j] = crt_out<OutT>(v, beta, beta != 0.0 ? dst[j] : OutT(0));
+    }
+    if (vec_c) {
+      constexpr int VE = 16 / (int)sizeof(OutT);
+#pragma unroll
+      for (int q = 0; q < 4 / VE; ++q) {
+        vec4u cv;
+        __builtin_memcpy(&cv, o + q * VE, 16);
+        *reinterpret_cast<vec4u *>(dst + q * VE) = cv;
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) dst[j] = o[j];
     }
   } else {
     int u[NMOD];
 #pragma unroll
     for (int t = 0; t < NMOD; ++t) u[t] = R[t * MN + idx];
-    crt_store(crt_value<NMOD>(u), ea[m], ce, shift, alpha, beta, dst);
+    const double v = crt_scaled(crt_value<NMOD>(u), ea[m], ce, shift, alpha);
+    *dst = crt_out<OutT>(v, beta, beta != 0.0 ? *dst : OutT(0));
   }
 }
 
@@ -827,8 +945,8 @@ void launch_crt_residues(hipStream_t s, int R, int K, const T *X, int64_t ld,
                          bool kcont, const int *exps, int bits, int8_t *P,
                          int64_t Kp) {
   dim3 g((unsigned)cdiv(Kp, 64), (unsigned)cdiv(R, 64));
-  crt_residue_kernel<T, NMOD><<<g, 256, 0, s>>>(R, K, X, ld, kcont, exps,
-                                               bits, P, Kp);
+  crt_residue_kernel<T, NMOD><<<g, 256, 0, s>>>(
+      R, K, X, ld, kcont, vec_aligned(X, ld), exps, bits, P, Kp);
 }
 template <int NMOD, typename OutT>
 void launch_crt_combine(hipStream_t s, int M, int N, const uint8_t *R,
@@ -836,13 +954,12 @@ void launch_crt_combine(hipStream_t s, int M, int N, const uint8_t *R,
                         double beta, OutT *C, int64_t ldc) {
   const int64_t MN = (int64_t)M * N;
   if (M % 4 == 0)
-    crt_combine_kernel<NMOD, OutT, true>
-        <<<(unsigned)cdiv(MN / 4, 256), 256, 0, s>>>(M, N, R, ea, eb, shift,
-                                                     alpha, beta, C, ldc);
+    crt_combine_kernel<NMOD, OutT, 4>
+        <<<(unsigned)cdiv(MN / 4, 256), 256, 0, s>>>(
+            M, N, R, ea, eb, shift, alpha, beta, C, ldc, vec_aligned(C, ldc));
   else
-    crt_combine_kernel<NMOD, OutT, false>
-        <<<(unsigned)cdiv(MN, 256), 256, 0, s>>>(M, N, R, ea, eb, shift,
-                                                 alpha, beta, C, ldc);
+    crt_combine_kernel<NMOD, OutT, 1><<<(unsigned)cdiv(MN, 256), 256, 0, s>>>(
+        M, N, R, ea, eb, shift, alpha, beta, C, ldc, false);
 }
 
 // Runtime modulus count -> template instance (8..20).
@@ -970,21 +1087,9 @@ hcblasStatus_t gemm_ozaki_int8(hcblasHandle_t h, const Canon &c, double alpha,
       const double *B = static_cast<const double *>(bv.B[z]) + c.bOff +
                         (c.b_k ? k0 : k0 * c.ldb);
       fill_int_kernel<<<(unsigned)cdiv(c.M + c.N, 256), 256, 0, h->stream>>>(
-          exps, c.M + c.N, INT_MIN);
-      if (c.a_k)
-        row_exp_kcont_kernel<<<dim3(c.M, (unsigned)cdiv(kc, 1024)), 256, 0,
-                               h->stream>>>(c.M, kc, A, c.lda, ea);
-      else
-        row_exp_rcont_kernel<<<dim3((unsigned)cdiv(c.M, 256),
-                                    (unsigned)cdiv(kc, 256)),
-                               256, 0, h->stream>>>(c.M, kc, A, c.lda, ea);
-      if (c.b_k)
-        row_exp_kcont_kernel<<<dim3(c.N, (unsigned)cdiv(kc, 1024)), 256, 0,
-                               h->stream>>>(c.N, kc, B, c.ldb, eb);
-      else
-        row_exp_rcont_kernel<<<dim3((unsigned)cdiv(c.N, 256),
-                                    (unsigned)cdiv(kc, 256)),
-                               256, 0, h->stream>>>(c.N, kc, B, c.ldb, eb);
+          exps, c.M + c.N, kNoScale);
+      launch_row_exp(h->stream, c.M, kc, A, c.lda, c.a_k, ea);
+      launch_row_exp(h->stream, c.N, kc, B, c.ldb, c.b_k, eb);
       dim3 ga((unsigned)cdiv(Kp, 32), (unsigned)cdiv(c.M, 32));
       ozaki_slice_kernel<<<ga, 256, 0, h->stream>>>(c.M, kc, A, c.lda, c.a_k,
                                                     ea, s, Ap, Kp, false);
@@ -1102,21 +1207,9 @@ hcblasStatus_t gemm_ozaki_crt(hcblasHandle_t h, const Canon &c, double alpha,
       const double *B = static_cast<const double *>(bv.B[z]) + c.bOff +
                         (c.b_k ? k0 : k0 * c.ldb);
       fill_int_kernel<<<(unsigned)cdiv(c.M + c.N, 256), 256, 0, h->stream>>>(
-          exps, c.M + c.N, INT_MIN);
-      if (c.a_k)
-        row_exp_kcont_kernel<<<dim3(c.M, (unsigned)cdiv(kc, 1024)), 256, 0,
-                               h->stream>>>(c.M, kc, A, c.lda, ea);
-      else
-        row_exp_rcont_kernel<<<dim3((unsigned)cdiv(c.M, 256),
-                                    (unsigned)cdiv(kc, 256)),
-                               256, 0, h->stream>>>(c.M, kc, A, c.lda, ea);
-      if (c.b_k)
-        row_exp_kcont_kernel<<<dim3(c.N, (unsigned)cdiv(kc, 1024)), 256, 0,
-                               h->stream>>>(c.N, kc, B, c.ldb, eb);
-      else
-        row_exp_rcont_kernel<<<dim3((unsigned)cdiv(c.N, 256),
-                                    (unsigned)cdiv(kc, 256)),
-                               256, 0, h->stream>>>(c.N, kc, B, c.ldb, eb);
+          exps, c.M + c.N, kNoScale);
+      launch_row_exp(h->stream, c.M, kc, A, c.lda, c.a_k, ea);
+      launch_row_exp(h->stream, c.N, kc, B, c.ldb, c.b_k, eb);
       if (!with_nmod<ResidueLaunch>(nmod, h->stream, c.M, kc, A, c.lda,
                                     c.a_k, (const int *)ea, bits, Ap, Kp) ||
           !with_nmod<ResidueLaunch>(nmod, h->stream, c.N, kc, B, c.ldb,
@@ -1142,6 +1235,113 @@ hcblasStatus_t gemm_ozaki_crt(hcblasHandle_t h, const Canon &c, double alpha,
       with_nmod<CombineLaunch>(nmod, h->stream, c.M, c.N, (const uint8_t *)R,
                                (const int *)ea, (const int *)eb, 2 * bits,
                                alpha, k0 == 0 ? beta : 1.0, Cz, ldc);
+      st = launch_status();
+      if (st != HCBLAS_STATUS_SUCCESS) return st;
+    }
+  }
+  return HCBLAS_STATUS_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
+// Ozaki scheme II for FP32 (S / C GEMM). Operands are scaled per row (A) and
+// per column (B) by their 2-norms instead of their largest element: with
+// ||A'_m||_2, ||B'_n||_2 <= 2^beta, Cauchy-Schwarz bounds every |(A'B')(m,n)|
+// by 2^(2 beta) < P / 2 for any K, and the operand precision of a row is its
+// norm's, which for rows with a wide exponent spread is several bits more
+// than the max-element scaling gives. beta = floor((log2 P - 1.01) / 2), at
+// most 30 so that the scaled operands fit one 32-bit limb. Default 8 moduli
+// (beta = 30): mean and median error below the FP32 kernel's for uniform and
+// wide-exponent inputs at K = 4096..8192.
+static int crt_beta_f32(int nmod) {
+  double log2p = 0.0;
+  for (int t = 0; t < nmod; ++t) log2p += std::log2((double)kModuli[t]);
+  const int b = (int)std::floor((log2p - 1.01) / 2.0);
+  return b > 30 ? 30 : b;
+}
+
+hcblasStatus_t gemm_ozaki_crt_f32(hcblasHandle_t h, const Canon &c,
+                                  float alpha, float beta, float *C,
+                                  int64_t ldc, int64_t cOff, int64_t strideC,
+                                  void *const *Cptrs, int64_t batch) {
+  if (c.M == 0 || c.N == 0 || batch == 0) return HCBLAS_STATUS_SUCCESS;
+  if (alpha == 0.f || c.K == 0)
+    return gemm_real<Elem::F32, float>(h, c, alpha, beta, C, ldc, cOff,
+                                       strideC, Cptrs, batch);
+  constexpr int BM = 256, BN = 256, BK = 128, WM = 2, WN = 4, MI = 32;
+  constexpr int64_t kmax = 65536;  // INT32 exactness: kc * 128 * 128 < 2^31
+  const int64_t kchunk = std::min<int64_t>(cdiv(c.K, BK) * BK, kmax);
+  const int64_t Kp = kchunk;
+  const int nmod = h->ozaki_moduli ? h->ozaki_moduli : 8;
+  const int bb = crt_beta_f32(nmod);
+  const int pa = (int)cdiv(kchunk, 64), pb = pa;  // 64-wide k blocks
+  const size_t a_bytes = (size_t)nmod * c.M * Kp;
+  const size_t b_bytes = (size_t)nmod * c.N * Kp;
+  const size_t r_bytes = (size_t)nmod * c.M * c.N;
+  const size_t e_bytes = sizeof(int) * ((size_t)c.M + c.N);
+  const size_t p_bytes = sizeof(double) * ((size_t)pa * c.M + (size_t)pb * c.N);
+  auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
+  if (ensure_workspace(h->split_ws, up(a_bytes) + up(b_bytes) + up(r_bytes) +
+                                        up(e_bytes) + p_bytes) != hipSuccess)
+    return HCBLAS_STATUS_ALLOC_FAILED;
+  char *ws = static_cast<char *>(h->split_ws.ptr);
+  int8_t *Ap = reinterpret_cast<int8_t *>(ws);
+  int8_t *Bp = reinterpret_cast<int8_t *>(ws + up(a_bytes));
+  uint8_t *R = reinterpret_cast<uint8_t *>(ws + up(a_bytes) + up(b_bytes));
+  int *exps = reinterpret_cast<int *>(ws + up(a_bytes) + up(b_bytes) +
+                                      up(r_bytes));
+  double *part = reinterpret_cast<double *>(ws + up(a_bytes) + up(b_bytes) +
+                                            up(r_bytes) + up(e_bytes));
+  int *ea = exps, *eb = exps + c.M;
+  double *parta = part, *partb = part + (size_t)pa * c.M;
+  BatchView bv;
+  hcblasStatus_t st =
+      batch_pointers(h, c, C, strideC, Cptrs, batch, sizeof(float), bv);
+  if (st != HCBLAS_STATUS_SUCCESS) return st;
+  auto norms = [&](int R, int kc, const float *X, int64_t ld, bool kcont,
+                   double *pp, int *ex) {
+    const int nb = (int)cdiv(kc, 64);
+    row_sumsq_kernel<<<dim3((unsigned)nb, (unsigned)cdiv(R, 64)), 256, 0,
+                       h->stream>>>(R, kc, X, ld, kcont, vec_aligned(X, ld),
+                                    pp);
+    row_norm_shift_kernel<<<(unsigned)cdiv(R, 4), 256, 0, h->stream>>>(
+        R, nb, pp, bb, kc, ex);
+  };
+  for (int64_t z = 0; z < batch; ++z) {
+    float *Cz = static_cast<float *>(bv.C[z]) + cOff;
+    for (int64_t k0 = 0; k0 < c.K; k0 += kchunk) {
+      const int kc = (int)std::min<int64_t>(kchunk, c.K - k0);
+      const float *A = static_cast<const float *>(bv.A[z]) + c.aOff +
+                       (c.a_k ? k0 : k0 * c.lda);
+      const float *B = static_cast<const float *>(bv.B[z]) + c.bOff +
+                       (c.b_k ? k0 : k0 * c.ldb);
+      norms(c.M, kc, A, c.lda, c.a_k, parta, ea);
+      norms(c.N, kc, B, c.ldb, c.b_k, partb, eb);
+      if (!with_nmod<ResidueLaunch>(nmod, h->stream, c.M, kc, A, c.lda,
+                                    c.a_k, (const int *)ea, 0, Ap, Kp) ||
+          !with_nmod<ResidueLaunch>(nmod, h->stream, c.N, kc, B, c.ldb,
+                                    c.b_k, (const int *)eb, 0, Bp, Kp))
+        return HCBLAS_STATUS_INVALID_VALUE;
+      st = launch_status();
+      if (st != HCBLAS_STATUS_SUCCESS) return st;
+      CrtParams p{};
+      p.M = c.M;
+      p.N = c.N;
+      p.Kp = Kp;
+      p.nmod = nmod;
+      p.Ap = Ap;
+      p.Bp = Bp;
+      p.R = R;
+      p.tiles_r = (int)cdiv(c.N, BM);
+      p.tiles_c = (int)cdiv(c.M, BN);
+      crt_gemm_kernel<BM, BN, BK, WM, WN, MI>
+          <<<dim3((unsigned)(p.tiles_r * p.tiles_c), 1, (unsigned)nmod),
+             64 * WM * WN, 0, h->stream>>>(p);
+      st = launch_status();
+      if (st != HCBLAS_STATUS_SUCCESS) return st;
+      with_nmod<CombineLaunch>(nmod, h->stream, c.M, c.N, (const uint8_t *)R,
+                               (const int *)ea, (const int *)eb, 0,
+                               (double)alpha, k0 == 0 ? (double)beta : 1.0,
+                               Cz, ldc);
       st = launch_status();
       if (st != HCBLAS_STATUS_SUCCESS) return st;
     }

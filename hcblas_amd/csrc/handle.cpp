@@ -119,7 +119,7 @@ hcblasStatus_t hcblasGetStream(hcblasHandle_t handle, hipStream_t *stream) {
 hcblasStatus_t hcblasSetMathMode(hcblasHandle_t handle, int mode) {
   if (!handle_ok(handle)) return HCBLAS_STATUS_NOT_INITIALIZED;
   if (mode & ~(HCBLAS_SPLIT_BF16_MATH | HCBLAS_OZAKI_INT8_MATH |
-               HCBLAS_OZAKI_CRT_MATH))
+               HCBLAS_OZAKI_CRT_MATH | HCBLAS_OZAKI_CRT_F32_MATH))
     return HCBLAS_STATUS_INVALID_VALUE;
   handle->math_mode = mode;
   return HCBLAS_STATUS_SUCCESS;

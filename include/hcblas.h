@@ -102,12 +102,18 @@ typedef uint16_t hcblasHalf;
  *                      modulus of a set of pairwise coprime moduli <= 256,
  *                      exact Chinese-remainder reconstruction in the
  *                      epilogue. Takes precedence over OZAKI_INT8.
- *   Modes combine with bitwise OR. */
+ *   OZAKI_CRT_F32    : S/C GEMM run as Ozaki scheme II on INT8 MFMA with
+ *                      operands scaled by their row / column 2-norms
+ *                      (8 moduli by default). Takes precedence over
+ *                      SPLIT_BF16.
+ *   Modes combine with bitwise OR. In the Ozaki modes a row of A or column
+ *   of B holding an Inf or NaN makes every output it feeds NaN. */
 typedef enum hcblasMathMode_t {
   HCBLAS_DEFAULT_MATH = 0,
   HCBLAS_SPLIT_BF16_MATH = 1,
   HCBLAS_OZAKI_INT8_MATH = 2,
-  HCBLAS_OZAKI_CRT_MATH = 4
+  HCBLAS_OZAKI_CRT_MATH = 4,
+  HCBLAS_OZAKI_CRT_F32_MATH = 8
 } hcblasMathMode_t;
 
 /* ------------------------------------------------------------------------ */
@@ -140,11 +146,12 @@ hcblasStatus_t hcblasGetMathMode(hcblasHandle_t handle, int *mode);
  * (no reference counterpart). Default 8. */
 hcblasStatus_t hcblasSetOzakiSlices(hcblasHandle_t handle, int slices);
 
-/* Number of moduli for the OZAKI_CRT path, 8..20, or 0 (the default) to
- * pick, per K, the fewest moduli whose operand precision matches the plain
- * FP64 kernel's error, also for inputs with a wide exponent spread:
- * b >= 59 - log2(K)/2 bits, i.e. 16 moduli (55 bits) at K = 8192 (no
- * reference counterpart). */
+/* Number of moduli for the OZAKI_CRT paths, 8..20, or 0 (the default).
+ * FP64: 0 picks, per K, the fewest moduli whose operand precision matches
+ * the plain FP64 kernel's error, also for inputs with a wide exponent
+ * spread: b >= 59 - log2(K)/2 bits, i.e. 16 moduli (55 bits) at K = 8192.
+ * FP32: 0 selects 8 moduli (row / column norms scaled to 2^30)
+ * (no reference counterpart). */
 hcblasStatus_t hcblasSetOzakiModuli(hcblasHandle_t handle, int moduli);
 
 /* ------------------------------------------------------------------------ */
@@ -485,6 +492,16 @@ hcblasStatus_t hcblas_sgemm_split_bf16(hcblasHandle_t handle,
                                        const float *beta, float *C,
                                        int64_t ldc, int64_t aOffset,
                                        int64_t bOffset, int64_t cOffset);
+hcblasStatus_t hcblas_sgemm_ozaki_crt(hcblasHandle_t handle,
+                                      hcblasOrder_t order,
+                                      hcblasOperation_t typeA,
+                                      hcblasOperation_t typeB, int M, int N,
+                                      int K, const float *alpha,
+                                      const float *A, int64_t lda,
+                                      const float *B, int64_t ldb,
+                                      const float *beta, float *C,
+                                      int64_t ldc, int64_t aOffset,
+                                      int64_t bOffset, int64_t cOffset);
 hcblasStatus_t hcblas_dgemm_ozaki_int8(hcblasHandle_t handle,
                                        hcblasOrder_t order,
                                        hcblasOperation_t typeA,

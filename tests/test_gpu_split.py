@@ -1,6 +1,7 @@
 """Split-precision GEMM on the GPU (no reference counterpart):
-BF16 three-way split SGEMM / CGEMM, Ozaki INT8 slice DGEMM / ZGEMM ("d") and
-Ozaki scheme II (CRT over INT8 moduli) DGEMM / ZGEMM ("dc").
+BF16 three-way split SGEMM / CGEMM ("s"), Ozaki scheme II on INT8 for SGEMM /
+CGEMM with norm-scaled operands ("sc"), Ozaki INT8 slice DGEMM / ZGEMM ("d")
+and Ozaki scheme II (CRT over INT8 moduli) DGEMM / ZGEMM ("dc").
 
 Checks:
   - bit-exact on the reference's integer recipes (test/unit/hcblas_sgemm.cpp
@@ -29,7 +30,8 @@ def _split(hc, handle, p, o, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C,
            ldc):
     api = hc.api
     dA, dB, dC = to_dev(A), to_dev(B), to_dev(C)
-    fn = {"s": api.hcblas_sgemm_split_bf16, "d": api.hcblas_dgemm_ozaki_int8,
+    fn = {"s": api.hcblas_sgemm_split_bf16, "sc": api.hcblas_sgemm_ozaki_crt,
+          "d": api.hcblas_dgemm_ozaki_int8,
           "dc": api.hcblas_dgemm_ozaki_crt}[p]
     st = fn(handle, o, ta, tb, M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc,
             0, 0, 0)
@@ -47,7 +49,7 @@ def _plain(hc, handle, p, o, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C,
     return to_host(dC)
 
 
-@pytest.mark.parametrize("p", ["s", "d", "dc"])
+@pytest.mark.parametrize("p", ["s", "sc", "d", "dc"])
 def test_reference_recipe_bit_exact(hc, handle, golden, p):
     g = golden("gemm_ref_int.npz")
     M, N, K = (int(v) for v in g["dims"])
@@ -73,10 +75,10 @@ def _rel_errors(orc, o, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
     return np.abs(got[idx] - ref) / np.maximum(np.abs(ref), 1e-300)
 
 
-@pytest.mark.parametrize("p", ["s", "d", "dc"])
+@pytest.mark.parametrize("p", ["s", "sc", "d", "dc"])
 @pytest.mark.parametrize("M,N,K", [(33, 17, 45), (128, 256, 512), (300, 129, 1000)])
 def test_all_cases_within_plain_bound(hc, handle, orc, p, M, N, K):
-    eps = np.finfo(np.float32 if p == "s" else np.float64).eps
+    eps = np.finfo(np.float32 if p[0] == "s" else np.float64).eps
     for o in (COL, ROW):
         for ta in (N_, T_):
             for tb in (N_, T_):
@@ -100,7 +102,7 @@ def test_all_cases_within_plain_bound(hc, handle, orc, p, M, N, K):
                 assert np.max(np.abs(got[idx] - ref)) <= tol, (o, ta, tb)
 
 
-@pytest.mark.parametrize("p", ["s", "d", "dc"])
+@pytest.mark.parametrize("p", ["s", "sc", "d", "dc"])
 @pytest.mark.parametrize("wide", [False, True])
 @pytest.mark.parametrize("ta", [N_, T_])
 def test_error_no_worse_than_plain(hc, handle, orc, p, wide, ta):
@@ -133,7 +135,8 @@ def test_math_mode_routes_plain_entries(hc, handle, orc):
     lib = hc.load()
     api = hc.api
     M, N, K = 200, 150, 300
-    for p, mode in (("s", 1), ("d", 2), ("dc", 4), ("dc", 6)):
+    for p, mode in (("s", 1), ("sc", 8), ("sc", 9), ("d", 2), ("dc", 4),
+                    ("dc", 6)):
         A = _gen(orc, p, M * K, 5, False)
         B = _gen(orc, p, K * N, 6, False)
         C = _gen(orc, p, M * N, 7, False)
@@ -150,7 +153,7 @@ def test_math_mode_routes_plain_entries(hc, handle, orc):
         assert lib.hcblasSetMathMode(handle.value, 0) == 0
 
 
-@pytest.mark.parametrize("p,mode", [("c", 1), ("z", 2), ("z", 4)])
+@pytest.mark.parametrize("p,mode", [("c", 1), ("c", 8), ("z", 2), ("z", 4)])
 def test_complex_split_modes(hc, handle, orc, p, mode):
     lib = hc.load()
     api = hc.api
@@ -241,27 +244,64 @@ def test_crt_moduli_control_accuracy(hc, handle, orc):
     assert lib.hcblasSetOzakiModuli(handle.value, 21) == 3
 
 
-def test_crt_zero_rows_and_k_chunks(hc, handle, orc):
+@pytest.mark.parametrize("p", ["dc", "sc"])
+def test_crt_zero_rows_and_k_chunks(hc, handle, orc, p):
     """All-zero rows / columns (no exponent) give exact zeros; K above the
     65536 chunk runs in two chunks with beta applied once."""
     M, N, K = 40, 24, 70000
-    A = orc.uniform_f64(M * K, 21)
-    B = orc.uniform_f64(K * N, 22)
+    gen = orc.uniform_f64 if p == "dc" else orc.uniform_f32
+    A = gen(M * K, 21)
+    B = gen(K * N, 22)
     A = A.reshape(K, M)
     A[:, 5] = 0.0  # row 5 of A (column-major M x K)
     A = A.reshape(-1)
     B = B.reshape(N, K)
     B[7, :] = 0.0  # column 7 of B
     B = B.reshape(-1)
-    C = orc.uniform_f64(M * N, 23)
-    got = _split(hc, handle, "dc", COL, N_, N_, M, N, K, 1.0, A, M, B, K, 0.5,
+    C = gen(M * N, 23)
+    got = _split(hc, handle, p, COL, N_, N_, M, N, K, 1.0, A, M, B, K, 0.5,
                  C, M)
     rows = np.repeat(np.arange(M), N)
     cols = np.tile(np.arange(N), M)
     ref = orc.gemm_sample_exact(COL, N_, N_, M, N, K, 1.0, A, M, B, K, 0.5, C,
                                 M, rows, cols)
     idx = rows + cols * M
-    assert np.max(np.abs(got[idx] - ref)) <= 4 * K * 2.0 ** -52
+    ulp = 2.0 ** -52 if p == "dc" else 2.0 ** -23
+    assert np.max(np.abs(got[idx] - ref)) <= 4 * K * ulp
     Cm = got.reshape(N, M)
-    assert np.array_equal(Cm[:, 5], 0.5 * C.reshape(N, M)[:, 5])
-    assert np.array_equal(Cm[7, :], 0.5 * C.reshape(N, M)[7, :])
+    half = (C * np.asarray(0.5, C.dtype)).reshape(N, M)
+    assert np.array_equal(Cm[:, 5], half[:, 5])
+    assert np.array_equal(Cm[7, :], half[7, :])
+
+
+@pytest.mark.parametrize("p", ["s", "sc", "d", "dc"])
+def test_non_finite_inputs_propagate(hc, handle, orc, p):
+    """A NaN or Inf in a row of A or a column of B makes the outputs of that
+    row / column non-finite (NaN in the Ozaki modes, where the operand scaling
+    has no finite value to work with); every other output keeps its
+    accuracy."""
+    M, N, K = 64, 48, 96
+    gen = orc.uniform_f32 if p[0] == "s" else orc.uniform_f64
+    A = gen(M * K, 31)
+    B = gen(K * N, 32)
+    A[3 + 10 * M] = np.nan  # A(3, 10), column-major M x K
+    A[7 + 20 * M] = np.inf  # A(7, 20)
+    B[5 + 2 * K] = -np.inf  # B(5, 2), column-major K x N
+    C = np.zeros(M * N, A.dtype)
+    got = _split(hc, handle, p, COL, N_, N_, M, N, K, 1.0, A, M, B, K, 0.0,
+                 C, M).reshape(N, M)
+    bad = np.zeros((N, M), bool)
+    bad[:, 3] = bad[:, 7] = True
+    bad[2, :] = True
+    assert not np.any(np.isfinite(got[bad]))
+    if p in ("sc", "d", "dc"):
+        assert np.all(np.isnan(got[bad]))
+    Af = A.copy()
+    Bf = B.copy()
+    Af[3 + 10 * M] = Af[7 + 20 * M] = 0.0
+    Bf[5 + 2 * K] = 0.0
+    cols, rows = np.nonzero(~bad)
+    ref = orc.gemm_sample_exact(COL, N_, N_, M, N, K, 1.0, Af, M, Bf, K, 0.0,
+                                C, M, rows, cols)
+    eps = np.finfo(A.dtype).eps
+    assert np.max(np.abs(got[cols, rows] - ref)) <= 4 * K * eps
