@@ -149,23 +149,23 @@ def time_bandwidth(torch, ops, steps, warmup):
     return step_ms, per_op, wall
 
 
-def gemm_section(hc, h, torch, oracle, n, reps=5):
-    """Plain vs split GEMM at n^3 (NN, column-major, uniform[-1,1])."""
+def gemm_section(hc, h, torch, oracle, n, reps=5, nc=4096):
+    """Plain vs split-precision GEMM (the precision configuration):
+    S / D GEMM at n^3, NN and TN (column-major), uniform[-1,1] and
+    sign * 2^u (u ~ U[-20, 20]); C / Z GEMM at nc^3, NN, uniform. Every
+    result carries its median time over `reps` calls, TFLOP/s at the
+    plain-equivalent 2MNK (8MNK complex), and max / mean relative error vs
+    the double-double oracle on 256 sampled entries; split results also
+    carry their speed ratio to the plain kernel on the same inputs."""
     import numpy as np
     api = hc.api
+    lib = hc.load()
     out = {}
     g = torch.Generator(device="cuda").manual_seed(7)
     rng = np.random.default_rng(0)
     ns = 256
-    rows = rng.integers(0, n, ns)
-    cols = rng.integers(0, n, ns)
 
-    def run(name, dt, fn, A, B, C, peak_key):
-        st = fn()
-        if st != 0:
-            out[name] = {"status": int(st)}
-            return
-        torch.cuda.synchronize()
+    def timed(fn):
         s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         times = []
         for _ in range(reps):
@@ -174,49 +174,113 @@ def gemm_section(hc, h, torch, oracle, n, reps=5):
             e.record()
             torch.cuda.synchronize()
             times.append(s.elapsed_time(e))
-        ms = sorted(times)[len(times) // 2]
-        tflops = 2.0 * n * n * n / (ms * 1e-3) / 1e12
-        # error vs double-double on sampled entries (C = A*B, col-major)
-        Ah = A.view(n, n)  # Ah[k, m] = A(m, k)
-        Bh = B.view(n, n)
-        Ch = C.view(n, n)
-        a_rows = Ah[:, torch.as_tensor(rows, device="cuda")].t().contiguous().cpu().numpy()
-        b_cols = Bh[torch.as_tensor(cols, device="cuda"), :].contiguous().cpu().numpy()
-        got = Ch[torch.as_tensor(cols, device="cuda"), torch.as_tensor(rows, device="cuda")].cpu().numpy()
-        ref = np.empty(ns)
-        for i in range(ns):
-            ref[i] = oracle.gemm_sample_exact(1, 0, 0, 1, 1, n, 1.0, a_rows[i].copy(), 1,
-                                              b_cols[i].copy(), n, 0.0,
-                                              np.zeros(1, a_rows.dtype), 1,
-                                              np.array([0]), np.array([0]))[0]
-        rel = np.abs(got.astype(np.float64) - ref) / np.maximum(np.abs(ref), 1e-300)
-        out[name] = {"tflops": round(tflops, 2), "ms": round(ms, 3),
-                     "peak_frac": round(tflops / PEAK_TFLOPS[peak_key], 4),
-                     "max_rel_err": float(rel.max()), "mean_rel_err": float(rel.mean())}
+        return sorted(times)[len(times) // 2]
 
-    for dt, tag, pk in ((torch.float32, "s", "fp32"), (torch.float64, "d", "fp64")):
-        A = torch.rand(n * n, generator=g, device="cuda", dtype=dt) * 2 - 1
-        B = torch.rand(n * n, generator=g, device="cuda", dtype=dt) * 2 - 1
-        C = torch.zeros(n * n, device="cuda", dtype=dt)
+    def errors(m, ta, A, B, C, dt_np):
+        # op(A)(r, :) and B(:, c) of sampled entries; column-major storage
+        rows = rng.integers(0, m, ns)
+        cols = rng.integers(0, m, ns)
+        Ah = A.view(m, m)   # Ah[j, i] = A_stored(i, j)
+        Bh = B.view(m, m)   # Bh[c, k] = B(k, c)
+        Ch = C.view(m, m)   # Ch[c, r] = C(r, c)
+        tr = torch.as_tensor(rows, device="cuda")
+        tc = torch.as_tensor(cols, device="cuda")
+        a_rows = (Ah[tr, :] if ta else Ah[:, tr].t()).contiguous().cpu().numpy()
+        b_cols = Bh[tc, :].contiguous().cpu().numpy()
+        got = Ch[tc, tr].cpu().numpy()
+        ref = np.empty(ns, np.complex128 if np.iscomplexobj(got) else np.float64)
+        zero = np.zeros(1, a_rows.dtype)
+        for i in range(ns):
+            ref[i] = oracle.gemm_sample_exact(1, 0, 0, 1, 1, m, 1.0, a_rows[i].copy(), 1,
+                                              b_cols[i].copy(), m, 0.0, zero, 1,
+                                              np.array([0]), np.array([0]))[0]
+        rel = np.abs(got.astype(ref.dtype) - ref) / np.maximum(np.abs(ref), 1e-300)
+        return float(rel.max()), float(rel.mean())
+
+    def result(ms, flop, peak_key, err):
+        tf = flop / (ms * 1e-3) / 1e12
+        return {"tflops": round(tf, 2), "ms": round(ms, 3),
+                "peak_frac": round(tf / PEAK_TFLOPS[peak_key], 4),
+                "max_rel_err": err[0], "mean_rel_err": err[1]}
+
+    def gen(m, dt, wide):
+        if not wide:
+            return torch.rand(m * m, generator=g, device="cuda", dtype=dt) * 2 - 1
+        u = torch.rand(m * m, generator=g, device="cuda", dtype=torch.float64) * 40 - 20
+        sg = torch.where(torch.rand(m * m, generator=g, device="cuda") < 0.5, -1.0, 1.0)
+        return (sg.double() * torch.exp2(u)).to(dt)
+
+    real = (("s", torch.float32, "fp32",
+             (("split_bf16", api.hcblas_sgemm_split_bf16),
+              ("ozaki_crt", api.hcblas_sgemm_ozaki_crt))),
+            ("d", torch.float64, "fp64",
+             (("ozaki_crt", api.hcblas_dgemm_ozaki_crt),
+              ("ozaki_int8", api.hcblas_dgemm_ozaki_int8))))
+    for tag, dt, pk, splits in real:
+        plain_fn = getattr(api, f"hcblas{tag.upper()}gemm")
+        for wide in (False, True):
+            A = gen(n, dt, wide)
+            B = gen(n, dt, wide)
+            C = torch.zeros(n * n, device="cuda", dtype=dt)
+            for ta in (0, 1):
+                key = f"{tag}gemm_{'tn' if ta else 'nn'}_{'wide' if wide else 'uniform'}"
+                ent = {}
+                fns = [("plain", lambda: plain_fn(h, ta, 0, n, n, n, 1.0, A, n, B, n, 0.0, C, n))]
+                for name, f in splits:
+                    fns.append((name, lambda f=f: f(h, 1, ta, 0, n, n, n, 1.0, A, n, B, n,
+                                                    0.0, C, n, 0, 0, 0)))
+                for name, fn in fns:
+                    st = fn()
+                    torch.cuda.synchronize()
+                    if st != 0:
+                        ent[name] = {"status": int(st)}
+                        continue
+                    ms = timed(fn)
+                    ent[name] = result(ms, 2.0 * n ** 3, pk, errors(n, ta, A, B, C, dt))
+                for name, _ in splits:
+                    if "ms" in ent.get(name, {}) and "ms" in ent["plain"]:
+                        ent[name]["vs_plain"] = round(ent["plain"]["ms"] / ent[name]["ms"], 3)
+                out[key] = ent
+            del A, B, C
+            torch.cuda.empty_cache()
+    # complex at nc^3 through the handle's math modes
+    for tag, dt, pk, modes in (("c", torch.complex64, "fp32", (("split_bf16", 1), ("ozaki_crt", 8))),
+                               ("z", torch.complex128, "fp64", (("ozaki_crt", 4), ("ozaki_int8", 2)))):
+        rdt = torch.float32 if tag == "c" else torch.float64
+        A = torch.complex(gen(nc, rdt, False), gen(nc, rdt, False))
+        B = torch.complex(gen(nc, rdt, False), gen(nc, rdt, False))
+        C = torch.zeros(nc * nc, device="cuda", dtype=dt)
         fn = getattr(api, f"hcblas{tag.upper()}gemm")
-        run(f"{tag}gemm_plain", dt, lambda: fn(h, 0, 0, n, n, n, 1.0, A, n, B, n, 0.0, C, n),
-            A, B, C, pk)
-        if tag == "s":
-            split = api.hcblas_sgemm_split_bf16
-            run("sgemm_split_bf16", dt,
-                lambda: split(h, 1, 0, 0, n, n, n, 1.0, A, n, B, n, 0.0, C, n, 0, 0, 0),
-                A, B, C, pk)
-        else:
-            oz = api.hcblas_dgemm_ozaki_int8
-            run("dgemm_ozaki_int8", dt,
-                lambda: oz(h, 1, 0, 0, n, n, n, 1.0, A, n, B, n, 0.0, C, n, 0, 0, 0),
-                A, B, C, pk)
-            crt = api.hcblas_dgemm_ozaki_crt
-            run("dgemm_ozaki_crt", dt,
-                lambda: crt(h, 1, 0, 0, n, n, n, 1.0, A, n, B, n, 0.0, C, n, 0, 0, 0),
-                A, B, C, pk)
+        call = lambda: fn(h, 0, 0, nc, nc, nc, 1.0 + 0j, A, nc, B, nc, 0j, C, nc)
+        ent = {}
+        for name, mode in (("plain", 0),) + modes:
+            lib.hcblasSetMathMode(h.value, mode)
+            st = call()
+            torch.cuda.synchronize()
+            if st != 0:
+                ent[name] = {"status": int(st)}
+                continue
+            ms = timed(call)
+            ent[name] = result(ms, 8.0 * nc ** 3, pk, errors(nc, 0, A, B, C, dt))
+        lib.hcblasSetMathMode(h.value, 0)
+        for name, _ in modes:
+            if "ms" in ent.get(name, {}) and "ms" in ent["plain"]:
+                ent[name]["vs_plain"] = round(ent["plain"]["ms"] / ent[name]["ms"], 3)
+        out[f"{tag}gemm_{nc}_nn_uniform"] = ent
         del A, B, C
         torch.cuda.empty_cache()
+    # headline ratios: plain-equivalent speed-up on NN uniform at n^3, and the
+    # worst case over the four S / D cases
+    def ratios(tag, name):
+        v = [out[k][name]["vs_plain"] for k in out
+             if k.startswith(f"{tag}gemm_") and name in out[k] and "vs_plain" in out[k][name]
+             and not k.startswith(f"{tag}gemm_{nc}")]
+        return {"nn_uniform": out[f"{tag}gemm_nn_uniform"][name].get("vs_plain"),
+                "min": min(v) if v else None}
+    out["speedup_vs_plain"] = {"sgemm_split_bf16": ratios("s", "split_bf16"),
+                               "sgemm_ozaki_crt": ratios("s", "ozaki_crt"),
+                               "dgemm_ozaki_crt": ratios("d", "ozaki_crt"),
+                               "dgemm_ozaki_int8": ratios("d", "ozaki_int8")}
     # 1024^3 row-major NN SGEMM (plain-SGEMM configuration)
     m = 1024
     A = torch.rand(m * m, generator=g, device="cuda") * 2 - 1
@@ -489,7 +553,8 @@ def main():
         import oracle
         del ops
         torch.cuda.empty_cache()
-        result["gemm"] = gemm_section(hc, h, torch, oracle, args.gemm_n)
+        result["gemm"] = gemm_section(hc, h, torch, oracle, args.gemm_n,
+                                      nc=args.gemm_n // 2)
     if not args.no_gemm:
         result["grouped"] = grouped_section(hc, h, torch)
         result["adjoint"] = adjoint_section(torch)

@@ -141,7 +141,7 @@ hcblasStatus_t hcblasSetOzakiSlices(hcblasHandle_t handle, int slices) {
 
 hcblasStatus_t hcblasSetOzakiModuli(hcblasHandle_t handle, int moduli) {
   if (!handle_ok(handle)) return HCBLAS_STATUS_NOT_INITIALIZED;
-  if (moduli != 0 && (moduli < 8 || moduli > 20))
+  if (moduli != 0 && (moduli < 6 || moduli > 20))
     return HCBLAS_STATUS_INVALID_VALUE;
   handle->ozaki_moduli = moduli;
   return HCBLAS_STATUS_SUCCESS;

@@ -146,12 +146,13 @@ hcblasStatus_t hcblasGetMathMode(hcblasHandle_t handle, int *mode);
  * (no reference counterpart). Default 8. */
 hcblasStatus_t hcblasSetOzakiSlices(hcblasHandle_t handle, int slices);
 
-/* Number of moduli for the OZAKI_CRT paths, 8..20, or 0 (the default).
+/* Number of moduli for the OZAKI_CRT paths, 6..20, or 0 (the default).
  * FP64: 0 picks, per K, the fewest moduli whose operand precision matches
  * the plain FP64 kernel's error, also for inputs with a wide exponent
  * spread: b >= 59 - log2(K)/2 bits, i.e. 16 moduli (55 bits) at K = 8192.
- * FP32: 0 selects 8 moduli (row / column norms scaled to 2^30)
- * (no reference counterpart). */
+ * FP32: 0 selects 8 moduli (row / column 2-norms scaled to 2^30, more
+ * accurate than the FP32 kernel also for wide exponent spreads; 7 moduli
+ * scale to 2^27, 6 to 2^23) (no reference counterpart). */
 hcblasStatus_t hcblasSetOzakiModuli(hcblasHandle_t handle, int moduli);
 
 /* ------------------------------------------------------------------------ */

@@ -240,8 +240,46 @@ def test_crt_moduli_control_accuracy(hc, handle, orc):
     assert errs[0] > errs[1] > errs[2]
     assert errs[3] <= errs[2]
     assert errs[2] <= ep
-    assert lib.hcblasSetOzakiModuli(handle.value, 7) == 3
+    assert lib.hcblasSetOzakiModuli(handle.value, 5) == 3
     assert lib.hcblasSetOzakiModuli(handle.value, 21) == 3
+
+
+@pytest.mark.parametrize("wide", [False, True])
+def test_crt_f32_moduli_control_accuracy(hc, handle, orc, wide):
+    """FP32 CRT: 6, 7, 8, 10 moduli scale the operand norms to 2^23, 2^27,
+    2^30, 2^30; the error falls with the first three and stays put at 10 (the
+    scale is capped at 2^30); the default (8) is below the FP32 kernel's
+    error in max, mean and median."""
+    lib = hc.load()
+    M = N = 192
+    K = 2048
+    gen = orc.wide_f32 if wide else orc.uniform_f32
+    A = gen(M * K, 41)
+    B = gen(K * N, 42)
+    C = np.zeros(M * N, np.float32)
+    rng = np.random.default_rng(4)
+    rows = rng.integers(0, M, 1024)
+    cols = rng.integers(0, N, 1024)
+
+    def err(got):
+        return _rel_errors(orc, COL, N_, N_, M, N, K, 1.0, A, M, B, K, 0.0, C,
+                           M, got, rows, cols)
+
+    errs = {}
+    for n in (6, 7, 8, 10, 0):
+        assert lib.hcblasSetOzakiModuli(handle.value, n) == 0
+        errs[n] = err(_split(hc, handle, "sc", COL, N_, N_, M, N, K, 1.0, A, M,
+                             B, K, 0.0, C, M))
+    assert lib.hcblasSetOzakiModuli(handle.value, 0) == 0
+    ep = err(_plain(hc, handle, "s", COL, N_, N_, M, N, K, 1.0, A, M, B, K,
+                    0.0, C, M))
+    print("crt f32 mean rel err by moduli:",
+          {n: float(e.mean()) for n, e in errs.items()}, "plain", ep.mean())
+    assert errs[6].mean() > errs[7].mean() > errs[8].mean()
+    assert errs[10].mean() <= 1.01 * errs[8].mean()
+    assert np.array_equal(errs[0], errs[8])
+    for f in (np.max, np.mean, np.median):
+        assert f(errs[0]) <= f(ep)
 
 
 @pytest.mark.parametrize("p", ["dc", "sc"])
