@@ -149,7 +149,7 @@ def time_bandwidth(torch, ops, steps, warmup):
     return step_ms, per_op, wall
 
 
-def gemm_section(hc, h, torch, oracle, n, reps=5, nc=4096):
+def gemm_section(hc, h, torch, oracle, n, reps=20, nc=4096):
     """Plain vs split-precision GEMM (the precision configuration):
     S / D GEMM at n^3, NN and TN (column-major), uniform[-1,1] and
     sign * 2^u (u ~ U[-20, 20]); C / Z GEMM at nc^3, NN, uniform. Every
@@ -163,7 +163,7 @@ def gemm_section(hc, h, torch, oracle, n, reps=5, nc=4096):
     out = {}
     g = torch.Generator(device="cuda").manual_seed(7)
     rng = np.random.default_rng(0)
-    ns = 256
+    ns = 4096
 
     def timed(fn):
         s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -177,7 +177,9 @@ def gemm_section(hc, h, torch, oracle, n, reps=5, nc=4096):
         return sorted(times)[len(times) // 2]
 
     def errors(m, ta, A, B, C, dt_np):
-        # op(A)(r, :) and B(:, c) of sampled entries; column-major storage
+        # op(A)(r, :) and B(:, c) of the sampled entries (column-major
+        # storage), then one oracle call on the ns x m / m x ns slices whose
+        # diagonal holds the sampled entries (beta = 0: C is not read)
         rows = rng.integers(0, m, ns)
         cols = rng.integers(0, m, ns)
         Ah = A.view(m, m)   # Ah[j, i] = A_stored(i, j)
@@ -188,12 +190,10 @@ def gemm_section(hc, h, torch, oracle, n, reps=5, nc=4096):
         a_rows = (Ah[tr, :] if ta else Ah[:, tr].t()).contiguous().cpu().numpy()
         b_cols = Bh[tc, :].contiguous().cpu().numpy()
         got = Ch[tc, tr].cpu().numpy()
-        ref = np.empty(ns, np.complex128 if np.iscomplexobj(got) else np.float64)
-        zero = np.zeros(1, a_rows.dtype)
-        for i in range(ns):
-            ref[i] = oracle.gemm_sample_exact(1, 0, 0, 1, 1, m, 1.0, a_rows[i].copy(), 1,
-                                              b_cols[i].copy(), m, 0.0, zero, 1,
-                                              np.array([0]), np.array([0]))[0]
+        diag = np.arange(ns)
+        ref = oracle.gemm_sample_exact(1, 1, 0, ns, ns, m, 1.0, a_rows.reshape(-1), m,
+                                       b_cols.reshape(-1), m, 0.0,
+                                       np.zeros(1, a_rows.dtype), ns, diag, diag)
         rel = np.abs(got.astype(ref.dtype) - ref) / np.maximum(np.abs(ref), 1e-300)
         return float(rel.max()), float(rel.mean())
 
@@ -300,6 +300,54 @@ def gemm_section(hc, h, torch, oracle, n, reps=5, nc=4096):
     ms = sorted(ts)[len(ts) // 2]
     out["sgemm_rowmajor_1024"] = {"tflops": round(2.0 * m ** 3 / (ms * 1e-3) / 1e12, 2),
                                   "ms": round(ms, 4)}
+    return out
+
+
+def convnet_section(hc, h, torch, reps=20):
+    """Column-major SGEMM over the 155 distinct shapes of the reference's
+    convolution-network lists (AlexNet, GoogLeNet, OverFeat, Oxford;
+    tests/golden/convnet_shapes.json): median time of `reps` calls per shape,
+    aggregate TF/s = sum 2MNK / sum of the medians, per list and overall."""
+    import json
+    api = hc.api
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests",
+                        "golden", "convnet_shapes.json")
+    with open(path) as f:
+        rows = [r for r in json.load(f)["shapes"] if r[13].endswith("BlasDimensions")]
+    per = {}
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    for r in rows:
+        M, N, K, ta, tb, lda, ldb, ldc = r[:8]
+        alpha, beta = r[8], r[9]
+        A = torch.rand(lda * (M if ta else K), device="cuda")
+        B = torch.rand(ldb * (K if tb else N), device="cuda")
+        C = torch.rand(ldc * N, device="cuda")
+        fn = lambda: api.hcblas_sgemm(h, 1, ta, tb, M, N, K, alpha, A, lda, B, ldb,
+                                      beta, C, ldc, 0, 0, 0)
+        fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            s.record()
+            fn()
+            e.record()
+            torch.cuda.synchronize()
+            ts.append(s.elapsed_time(e))
+        ms = sorted(ts)[len(ts) // 2]
+        agg = per.setdefault(r[13].replace("BlasDimensions", "").lower(),
+                             {"shapes": 0, "flop": 0.0, "ms": 0.0})
+        agg["shapes"] += 1
+        agg["flop"] += 2.0 * M * N * K
+        agg["ms"] += ms
+        del A, B, C
+    tot_f = sum(v["flop"] for v in per.values())
+    tot_ms = sum(v["ms"] for v in per.values())
+    out = {"shapes": sum(v["shapes"] for v in per.values()),
+           "tflops": round(tot_f / (tot_ms * 1e-3) / 1e12, 2),
+           "ms_sum": round(tot_ms, 3), "per_list": {}}
+    for k, v in per.items():
+        out["per_list"][k] = {"shapes": v["shapes"], "ms_sum": round(v["ms"], 3),
+                              "tflops": round(v["flop"] / (v["ms"] * 1e-3) / 1e12, 2)}
     return out
 
 
@@ -555,6 +603,7 @@ def main():
         torch.cuda.empty_cache()
         result["gemm"] = gemm_section(hc, h, torch, oracle, args.gemm_n,
                                       nc=args.gemm_n // 2)
+        result["convnet"] = convnet_section(hc, h, torch)
     if not args.no_gemm:
         result["grouped"] = grouped_section(hc, h, torch)
         result["adjoint"] = adjoint_section(torch)
