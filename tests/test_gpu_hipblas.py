@@ -116,9 +116,10 @@ def test_level1_and_level2_forward(shim, hh, hc, handle, orc, torch_cuda):
     yv = orc.uniform_f32(M, 8)
     dA = to_dev(A)
     g1, g2 = to_dev(yv), to_dev(yv)
+    dxg = to_dev(xv)  # kept alive until the kernel ran
     b = ctypes.c_float(0.5)
     assert shim.hipblasSgemv(hh, 111, M, N, ctypes.byref(a), _p(dA), M,
-                             _p(to_dev(xv)), 1, ctypes.byref(b), _p(g1), 1) == 0
+                             _p(dxg), 1, ctypes.byref(b), _p(g1), 1) == 0
     assert api.hcblasSgemv(handle, 0, M, N, 0.75, dA, M, to_dev(xv), 1, 0.5,
                            g2, 1) == 0
     assert bitwise_equal(to_host(g1), to_host(g2))
