@@ -68,7 +68,11 @@ hcblasStatus_t launch_tiles(hipStream_t stream, GemmParams p, bool a_k,
 // grid still covers every CU is taken):
 //   FP32  256x256x32, 8 waves of 128x64 on 32x32 MFMA blocks (134 TF/s,
 //         85% of peak), then 256x128 8 waves (128), 128x128 4 waves, 64x64;
-//   FP64  128x64x16, 8 waves (60 TF/s, 77%), then 64x64 4 waves (59);
+//   FP64  256x128x16, 8 waves of 64x64 (66.9 TF/s NN / 64.5 TN / 66.9 NT /
+//         66.1 TT, 82-85%; 128x256 2x4 is as fast on NN, 128x128 and
+//         smaller tiles stay at 75% or below; aligned operands only, the
+//         element-wise loaders spill at this size), then 128x64x16 8 waves
+//         (58.5), then 64x64 4 waves;
 //   F16 / BF16 256x256x64, 8 waves on 32x32 blocks, then 128x128 4 waves,
 //         then 64x64.
 template <Elem E, typename OutT>
@@ -88,6 +92,9 @@ hcblasStatus_t launch_gemm(hipStream_t stream, int num_cus, GemmParams p,
       return launch_tiles<E, OutT, 128, 128, 32, 2, 2>(stream, p, a_k, b_k,
                                                        vload, batch);
   } else if constexpr (E == Elem::F64) {
+    if (vload && tiles(256, 128) >= num_cus)
+      return launch_tiles<E, OutT, 256, 128, 16, 4, 2>(stream, p, a_k, b_k,
+                                                       vload, batch);
     if (tiles(128, 64) >= num_cus)
       return launch_tiles<E, OutT, 128, 64, 16, 2, 4>(stream, p, a_k, b_k,
                                                       vload, batch);
