@@ -11,11 +11,13 @@
 //     (problem, tile) entries is ordered longest-first (LPT: tile cost ~ K,
 //     fp32 tiles weighted by the FP32:FP16 MFMA rate ratio) or in table order;
 //   - the persistent launch is captured into a hipGraph.
-// Run: the graph launch. Workgroups (a few per CU) take the next work entry
-// from a device counter; the last workgroup to finish resets the counters in
-// the kernel, so the graph holds a single kernel node and no memset nodes.
-// Autotune: times every (tile shape, order) candidate on the plan's own
-// problems and keeps the fastest.
+// Run: the graph launch. Workgroups (a few per CU) take work entries from a
+// device counter or by a static stride; the last workgroup to finish resets
+// the counters in the kernel, so the graph holds a single kernel node and no
+// memset nodes.
+// Autotune: times every (tile shape, order, workgroups per CU, queue or
+// static schedule) candidate on the plan's own problems and keeps the
+// fastest.
 //
 // Each tile runs gemm_tile, the same device code as the plain GEMM with the
 // same K step, so a grouped result is bit-identical to per-problem
@@ -90,24 +92,36 @@ __device__ __forceinline__ void run_tile(const ProbDev &p, int m0, int n0,
   }
 }
 
-// counters[0]: next work entry; counters[1]: finished workgroups.
-template <int TM, bool VL>
-__global__ __launch_bounds__(256) void grouped_gemm_kernel(
+// Dynamic schedule: counters[0] is the next work entry, taken with one
+// atomic per tile. Static schedule: workgroup g takes entries g, g + G,
+// g + 2G, ... (G = grid size); over a longest-first list that balances like
+// the queue without its atomic per tile. counters[1] counts finished
+// workgroups; the last one resets both counters.
+// OCC: workgroups per CU the launch is sized for (the register budget
+// follows from it: 4 waves per workgroup, one per SIMD).
+template <int TM, bool VL, int OCC>
+__global__ __launch_bounds__(256, OCC) void grouped_gemm_kernel(
     const ProbDev *__restrict__ probs, const int2 *__restrict__ work,
-    int total, unsigned int *counters) {
+    int total, unsigned int *counters, int dynamic) {
   __shared__ __attribute__((aligned(16))) char lds[GroupCfg<TM>::LDS];
   __shared__ int next;
-  for (;;) {
-    if (threadIdx.x == 0)
-      next = (int)__hip_atomic_fetch_add(&counters[0], 1u, __ATOMIC_RELAXED,
-                                         __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    const int w = next;
-    __syncthreads();
+  for (int s = blockIdx.x;; s += gridDim.x) {
+    int w = s;
+    if (dynamic) {
+      if (threadIdx.x == 0)
+        next = (int)__hip_atomic_fetch_add(&counters[0], 1u, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+      __syncthreads();
+      w = next;
+      __syncthreads();
+    }
+    // uniform across the workgroup: the descriptor comes through scalar loads
+    w = __builtin_amdgcn_readfirstlane(w);
     if (w >= total) break;
     const int2 it = work[w];
-    const ProbDev p = probs[it.x];
-    const int tm = it.y / p.tiles_n, tn = it.y - tm * p.tiles_n;
+    const ProbDev p = probs[__builtin_amdgcn_readfirstlane(it.x)];
+    const int t = __builtin_amdgcn_readfirstlane(it.y);
+    const int tm = t / p.tiles_n, tn = t - tm * p.tiles_n;
     if (p.dtype == 0)
       run_tile<Elem::F32, float, TM, VL>(p, tm * TM, tn * TM, lds);
     else
@@ -127,7 +141,8 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(
 
 constexpr int kTileShapes[] = {64, 32};
 constexpr int kNumShapes = 2;
-constexpr int kWgPerCu = 2;
+constexpr int kOccupancies[] = {2, 3, 4};  // workgroups per CU
+constexpr int kNumOcc = 3;
 
 }  // namespace
 }  // namespace hcb
@@ -147,6 +162,8 @@ struct hcblasGroupedPlan {
   int grid = 0;
   int shape = 0;   // index into kTileShapes
   int order = 0;   // 0: longest first, 1: table order
+  int occ = 0;     // index into kOccupancies
+  int sched = 0;   // 0: atomic queue, 1: static stride
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -160,29 +177,44 @@ void free_graph(hcblasGroupedPlan *p) {
   p->graph = nullptr;
 }
 
-void launch_grouped(hcblasGroupedPlan *p, hipStream_t s) {
-  const int tm = kTileShapes[p->shape];
+template <int TM, bool VL>
+void launch_occ(hcblasGroupedPlan *p, hipStream_t s) {
   dim3 g(p->grid);
-  if (tm == 64) {
-    if (p->vload)
-      grouped_gemm_kernel<64, true><<<g, 256, 0, s>>>(p->d_probs, p->d_work,
-                                                      p->total, p->d_counters);
-    else
-      grouped_gemm_kernel<64, false><<<g, 256, 0, s>>>(
-          p->d_probs, p->d_work, p->total, p->d_counters);
-  } else {
-    if (p->vload)
-      grouped_gemm_kernel<32, true><<<g, 256, 0, s>>>(p->d_probs, p->d_work,
-                                                      p->total, p->d_counters);
-    else
-      grouped_gemm_kernel<32, false><<<g, 256, 0, s>>>(
-          p->d_probs, p->d_work, p->total, p->d_counters);
+  switch (kOccupancies[p->occ]) {
+    case 2:
+      grouped_gemm_kernel<TM, VL, 2><<<g, 256, 0, s>>>(
+          p->d_probs, p->d_work, p->total, p->d_counters, p->sched == 0);
+      break;
+    case 3:
+      grouped_gemm_kernel<TM, VL, 3><<<g, 256, 0, s>>>(
+          p->d_probs, p->d_work, p->total, p->d_counters, p->sched == 0);
+      break;
+    default:
+      grouped_gemm_kernel<TM, VL, 4><<<g, 256, 0, s>>>(
+          p->d_probs, p->d_work, p->total, p->d_counters, p->sched == 0);
+      break;
   }
 }
 
-// Builds the work list for (shape, order), uploads it and captures the graph.
+void launch_grouped(hcblasGroupedPlan *p, hipStream_t s) {
+  const int tm = kTileShapes[p->shape];
+  if (tm == 64) {
+    if (p->vload)
+      launch_occ<64, true>(p, s);
+    else
+      launch_occ<64, false>(p, s);
+  } else {
+    if (p->vload)
+      launch_occ<32, true>(p, s);
+    else
+      launch_occ<32, false>(p, s);
+  }
+}
+
+// Builds the work list for (shape, order, occupancy), uploads it and
+// captures the graph.
 hcblasStatus_t build(hcblasHandle_t h, hcblasGroupedPlan *p, int shape,
-                     int order) {
+                     int order, int occ, int sched) {
   const int tm = kTileShapes[shape];
   std::vector<int2> work;
   std::vector<int64_t> cost;
@@ -224,7 +256,10 @@ hcblasStatus_t build(hcblasHandle_t h, hcblasGroupedPlan *p, int shape,
     return HCBLAS_STATUS_EXECUTION_FAILED;
   p->shape = shape;
   p->order = order;
-  p->grid = std::max(1, std::min(p->total, h->num_cus * kWgPerCu));
+  p->occ = occ;
+  p->sched = sched;
+  p->grid =
+      std::max(1, std::min(p->total, h->num_cus * kOccupancies[occ]));
   free_graph(p);
   if (p->total == 0) return HCBLAS_STATUS_SUCCESS;
   hipStream_t cs;
@@ -326,7 +361,7 @@ hcblasStatus_t hcblasGroupedPlanCreate(hcblasHandle_t handle,
     hcblasGroupedPlanDestroy(p);
     return HCBLAS_STATUS_ALLOC_FAILED;
   }
-  hcblasStatus_t st = build(handle, p, 0, 0);
+  hcblasStatus_t st = build(handle, p, 0, 0, 0, 0);
   if (st != HCBLAS_STATUS_SUCCESS) {
     hcblasGroupedPlanDestroy(p);
     return st;
@@ -353,36 +388,42 @@ hcblasStatus_t hcblasGroupedPlanAutotune(hcblasHandle_t handle,
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
     return HCBLAS_STATUS_EXECUTION_FAILED;
   float best = 1e30f;
-  int best_shape = 0, best_order = 0;
+  int best_shape = 0, best_order = 0, best_occ = 0, best_sched = 0;
   hcblasStatus_t st = HCBLAS_STATUS_SUCCESS;
   for (int shape = 0; shape < kNumShapes && st == HCBLAS_STATUS_SUCCESS;
        ++shape) {
-    for (int order = 0; order < 2; ++order) {
-      st = build(handle, plan, shape, order);
-      if (st != HCBLAS_STATUS_SUCCESS) break;
-      if (plan->total == 0) break;
-      (void)hipGraphLaunch(plan->exec, handle->stream);  // warm-up
-      (void)hipEventRecord(e0, handle->stream);
-      for (int r = 0; r < reps; ++r)
-        (void)hipGraphLaunch(plan->exec, handle->stream);
-      (void)hipEventRecord(e1, handle->stream);
-      if (hipEventSynchronize(e1) != hipSuccess) {
-        st = HCBLAS_STATUS_EXECUTION_FAILED;
-        break;
-      }
-      float ms = 0.f;
-      (void)hipEventElapsedTime(&ms, e0, e1);
-      if (ms < best) {
-        best = ms;
-        best_shape = shape;
-        best_order = order;
+    for (int order = 0; order < 2 && st == HCBLAS_STATUS_SUCCESS; ++order) {
+      for (int occ = 0; occ < kNumOcc && st == HCBLAS_STATUS_SUCCESS; ++occ) {
+        for (int sched = 0; sched < 2; ++sched) {
+          st = build(handle, plan, shape, order, occ, sched);
+          if (st != HCBLAS_STATUS_SUCCESS) break;
+          if (plan->total == 0) break;
+          (void)hipGraphLaunch(plan->exec, handle->stream);  // warm-up
+          (void)hipEventRecord(e0, handle->stream);
+          for (int r = 0; r < reps; ++r)
+            (void)hipGraphLaunch(plan->exec, handle->stream);
+          (void)hipEventRecord(e1, handle->stream);
+          if (hipEventSynchronize(e1) != hipSuccess) {
+            st = HCBLAS_STATUS_EXECUTION_FAILED;
+            break;
+          }
+          float ms = 0.f;
+          (void)hipEventElapsedTime(&ms, e0, e1);
+          if (ms < best) {
+            best = ms;
+            best_shape = shape;
+            best_order = order;
+            best_occ = occ;
+            best_sched = sched;
+          }
+        }
       }
     }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (st != HCBLAS_STATUS_SUCCESS) return st;
-  return build(handle, plan, best_shape, best_order);
+  return build(handle, plan, best_shape, best_order, best_occ, best_sched);
 }
 
 hcblasStatus_t hcblasGroupedPlanInfo(hcblasGroupedPlan_t plan, int *tile,
