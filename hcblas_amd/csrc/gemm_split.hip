@@ -1164,8 +1164,11 @@ hcblasStatus_t gemm_ozaki_crt(hcblasHandle_t h, const Canon &c, double alpha,
   if (alpha == 0.0 || c.K == 0)
     return gemm_real<Elem::F64, double>(h, c, alpha, beta, C, ldc, cOff,
                                         strideC, Cptrs, batch);
-  // 256x256 tiles, 8 waves on 16x16 INT8 MFMA blocks (38.5% of INT8 peak
-  // at 8192^2 x 16 moduli, against 28.8% for 128x128).
+  // 256x256 tiles, 8 waves of 128x64 on 32x32x32 INT8 MFMA blocks (8.4 ms
+  // for 8192^3 x 16 moduli, 42% of INT8 peak). Reading the next k chunk's
+  // fragments ahead of the current MFMAs (a second fragment set, 237 VGPRs)
+  // measured no faster (8.5 ms): with two waves per SIMD the LDS latency is
+  // already covered.
   constexpr int BM = 256, BN = 256, BK = 128, WM = 2, WN = 4, MI = 32;
   // INT32 exactness of one modulus product: kc * 128 * 128 < 2^31.
   constexpr int64_t kmax = 65536;
