@@ -106,6 +106,66 @@ def test_ger_bit_exact(hc, handle, orc, p, order, M, N):
         assert bitwise_equal(to_host(dA), exp)
 
 
+# Shapes that reach the band kernels: (p, order, trans, M, N, batch).
+# Column-contiguous case (col-major T, row-major N): bands of 16 or 8 row
+# steps of 256 x 16 bytes with enough column blocks of 8 to fill the GPU.
+# Row-contiguous case (col-major N, row-major T): bands of 4 row steps, at
+# least 64 columns, the columns split over blocks of >= 256. The single
+# cases have a partial last band.
+BAND_CASES = [("s", COL, T_, 18384, 4096, 1), ("s", ROW, N_, 4096, 9192, 1),
+              ("d", COL, T_, 12000, 4096, 1), ("d", ROW, N_, 8192, 5000, 1),
+              ("s", COL, T_, 16384, 64, 128),
+              ("s", COL, N_, 6000, 3000, 1), ("d", ROW, T_, 1500, 5000, 1),
+              ("d", COL, N_, 3000, 700, 1), ("s", COL, N_, 4096, 256, 16)]
+
+
+@pytest.mark.parametrize("case", BAND_CASES, ids=lambda c: "-".join(map(str, c)))
+def test_gemv_band_shapes(hc, handle, orc, case):
+    p, order, trans, M, N, b = case
+    api = hc.api
+    dt = np.float32 if p == "s" else np.float64
+    gen = orc.uniform_f32 if p == "s" else orc.uniform_f64
+    rows = M if order == COL else N
+    cols = N if order == COL else M
+    lenx, leny = (N, M) if trans == N_ else (M, N)
+    eps = np.finfo(dt).eps
+    if b > 1:
+        A = gen(M * N * b, 11)
+        x, y = gen(lenx * b, 12), gen(leny * b, 13)
+        dA, dx, dy = to_dev(A), to_dev(x), to_dev(y)
+        st = getattr(api, f"hcblas{p.upper()}gemvBatched")(
+            handle, trans, M, N, 0.5, dA, M, dx, 1, 2.0, dy, 1, b)
+        assert st == 0
+        got = to_host(dy)
+        for i in range(b):
+            ref = orc.gemv_exact(COL, trans, M, N, 0.5, A[i * M * N:], M,
+                                 x[i * lenx:], 1, 2.0, y[i * leny:], 1)
+            tol = 2 * (lenx + 2) * eps * (0.5 * lenx + 3)
+            assert np.max(np.abs(got[i * leny:(i + 1) * leny] - ref)) <= tol
+        return
+    lda = rows + 4
+    A = gen(lda * cols, M + N)
+    for incx, incy, alpha, beta in ((1, 1, 1.0, 0.0), (2, -1, 1.5, 0.5)):
+        x = gen(lenx * incx, 3)
+        y = gen(leny * abs(incy), 4)
+        yy = y.copy()
+        if beta == 0.0:
+            yy[:] = np.nan  # must not propagate
+        dA, dx, dy = to_dev(A), to_dev(x), to_dev(yy)
+        st = getattr(api, f"hcblas_{p}gemv")(
+            handle, order, trans, M, N, alpha, dA, 0, lda, dx, 0, incx, beta,
+            dy, 0, incy)
+        assert st == 0
+        got = to_host(dy)
+        ref = orc.gemv_exact(order, trans, M, N, alpha, A, lda, x, incx, beta,
+                             yy, incy)
+        idx = (np.arange(leny) * incy if incy > 0 else
+               (leny - 1 - np.arange(leny)) * (-incy))
+        tol = 2 * (lenx + 2) * eps * (abs(alpha) * lenx + abs(beta) + 1)
+        assert np.all(np.isfinite(got[idx]))
+        assert np.max(np.abs(got[idx] - ref)) <= tol, (incx, incy)
+
+
 @pytest.mark.parametrize("p", ["s", "d"])
 def test_batched(hc, handle, orc, p):
     api = hc.api
