@@ -598,24 +598,6 @@ __device__ __forceinline__ constexpr int modulus() {
   return m[T];
 }
 
-// Symmetric residue of an integer s (|s| < 2^30) modulo M, as an int whose
-// low byte is the int8 residue: in [-(M-1)/2, (M-1)/2] for odd M, and for
-// M = 256 s itself (its low byte is s mod 256 in two's complement). The first
-// float quotient is within 2 of s/M (float(s) is off by at most 64, 1/M and
-// the product by a relative 2^-24 each), so |r| < 2.5 M; the second pass on
-// that small exact value is exact.
-template <int M>
-__device__ __forceinline__ int sym_residue(int s) {
-  if constexpr (M == 256) {
-    return s;
-  } else {
-    constexpr float inv = 1.0f / (float)M;
-    const int q = (int)__builtin_rintf((float)s * inv);
-    const float rf = (float)(s - q * M);
-    return (int)__builtin_fmaf(-__builtin_rintf(rf * inv), (float)M, rf);
-  }
-}
-
 // Low bytes of four ints packed into one dword.
 __device__ __forceinline__ uint32_t pack_bytes(int b0, int b1, int b2, int b3) {
   const uint32_t lo = __builtin_amdgcn_perm((uint32_t)b1, (uint32_t)b0,
@@ -625,31 +607,58 @@ __device__ __forceinline__ uint32_t pack_bytes(int b0, int b1, int b2, int b3) {
   return __builtin_amdgcn_perm(hi, lo, 0x05040100u);
 }
 
-// Signed limbs of the integers v_j = l2 2^42 + l1 2^21 + l0 (|limb| < 2^21,
-// all limbs of v_j share its sign; WIDE = false: |v_j| < 2^30 and only l0 is
-// used).
+// Residue inputs of 16 integers v_j, as unsigned words whose byte expansion
+// is congruent to v_j up to a known offset:
+//   FP32 (|v| <= 2^30):  lo = v + 2^30                      (v = lo - 2^30)
+//   FP64 (|v| <= 2^62):  lo = v mod 2^32, hi = floor(v / 2^32) + 2^30
+//                                            (v = lo + hi 2^32 - 2^62)
 template <bool WIDE>
-struct Limbs16 {
-  int l2[WIDE ? 16 : 1], l1[WIDE ? 16 : 1], l0[16];
+struct ResidueIn {
+  uint32_t lo[16], hi[WIDE ? 16 : 1];
+};
+
+// Byte weights and offset of modulus M: the dot product of the input bytes
+// with (2^(8i) mod M) plus k, where k = -offset mod M, is a nonnegative
+// integer below 2^19 congruent to v modulo M.
+template <int M, bool WIDE>
+struct ResidueWeights {
+  static constexpr uint32_t w(int shift) {
+    uint32_t r = 0;
+    for (int i = 0; i < 4; ++i)
+      r |= (uint32_t)((1ull << (shift + 8 * i)) % M) << (8 * i);
+    return r;
+  }
+  static constexpr uint32_t clo = w(0), chi = w(32);
+  static constexpr uint32_t k =
+      (uint32_t)((M - (1ull << (WIDE ? 62 : 30)) % M) % M);
 };
 
 // Residue planes of the 16 integers of one row segment: one 16-byte store
-// per modulus plane (plane t of the row at row + t * Kp).
+// per modulus plane (plane t of the row at row + t * Kp). Each stored byte is
+// the symmetric residue of v modulo m_t, in [-(m-1)/2, (m-1)/2] for odd m
+// and the two's-complement low byte of v for m = 256. For odd m, s (the
+// weighted byte sum, s < 2^19) is exact in float, rint(s * (1/m)) is the
+// nearest integer to s/m (the product's error, below 2^-4/m, is smaller than
+// the distance 1/(2m) of s/m from a half-integer), and s - q*m is exact.
 template <int NMOD, bool WIDE, int T = 0>
-__device__ __forceinline__ void write_residues16(const Limbs16<WIDE> &L,
+__device__ __forceinline__ void write_residues16(const ResidueIn<WIDE> &in,
                                                  int8_t *row, int64_t Kp) {
   if constexpr (T < NMOD) {
     constexpr int M = modulus<T>();
-    constexpr int c21 = (int)((1ll << 21) % M);
-    constexpr int c42 = (int)((1ll << 42) % M);
     int q[16];
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
-      int sv = L.l0[j];
-      if constexpr (WIDE) {
-        if constexpr (M != 256) sv += L.l2[j] * c42 + L.l1[j] * c21;
+      if constexpr (M == 256) {
+        q[j] = (int)in.lo[j];  // v and lo agree modulo 2^8
+      } else {
+        using W = ResidueWeights<M, WIDE>;
+        uint32_t acc = W::k;
+        if constexpr (WIDE) acc = __builtin_amdgcn_udot4(in.hi[j], W::chi, acc, false);
+        acc = __builtin_amdgcn_udot4(in.lo[j], W::clo, acc, false);
+        constexpr float inv = 1.0f / (float)M;
+        const float sf = (float)acc;
+        q[j] = (int)__builtin_fmaf(-__builtin_rintf(sf * inv), (float)M, sf);
       }
-      q[j] = sym_residue<M>(sv);
     }
     vec4u w;
     w.x = pack_bytes(q[0], q[1], q[2], q[3]);
@@ -657,16 +666,17 @@ __device__ __forceinline__ void write_residues16(const Limbs16<WIDE> &L,
     w.z = pack_bytes(q[8], q[9], q[10], q[11]);
     w.w = pack_bytes(q[12], q[13], q[14], q[15]);
     *reinterpret_cast<vec4u *>(row + (int64_t)T * Kp) = w;
-    write_residues16<NMOD, WIDE, T + 1>(L, row, Kp);
+    write_residues16<NMOD, WIDE, T + 1>(in, row, Kp);
   }
 }
 
-// Residue planes P[r][t][0..Kp) of rint(X(r, k) * 2^(bits - e_r)) (Kp a
+// Residue planes P[r][t][0..Kp) of v = rint(X(r, k) * 2^(bits - e_r)) (Kp a
 // multiple of 64; zeros past K). A workgroup stages a 64 x 64 tile through
 // LDS, reading along the source's contiguous dimension; each thread then
 // converts 16 consecutive k of one row and writes one 16-byte vector per
-// plane. FP64 input: |v| <= 2^62 in three 21-bit limbs; FP32 input: bits
-// <= 30, so |v| <= 2^30 is one limb.
+// plane. FP32 input: bits - e_r keeps |v| <= 2^30, and the scaling and
+// rounding are exact in float (values below 2^-126 round to 0 either way);
+// FP64 input: |v| <= 2^62, split exactly into 32-bit halves in double.
 template <typename T, int NMOD>
 __global__ __launch_bounds__(256) void crt_residue_kernel(
     int R, int K, const T *__restrict__ X, int64_t ld, bool kcont, bool vec,
@@ -681,28 +691,25 @@ __global__ __launch_bounds__(256) void crt_residue_kernel(
   const int r = r0 + row;
   if (r >= R) return;
   const int e = exps[r];
-  Limbs16<WIDE> L;
+  const bool zero = e == kNoScale || e == kNonFinite;
+  ResidueIn<WIDE> in;
 #pragma unroll
   for (int j = 0; j < 16; ++j) {
-    const double v =
-        (e == kNoScale || e == kNonFinite)
-            ? 0.0
-            : __builtin_rint(__builtin_ldexp((double)st.v[row][kq + j], bits - e));
     if constexpr (WIDE) {
-      const double a = __builtin_fabs(v);
-      const double a2 = __builtin_floor(a * 0x1p-42);
-      const double rem = a - a2 * 0x1p42;
-      const double a1 = __builtin_floor(rem * 0x1p-21);
-      const double a0 = rem - a1 * 0x1p21;
-      const int sg = v < 0.0 ? -1 : 1;
-      L.l2[j] = sg * (int)a2;
-      L.l1[j] = sg * (int)a1;
-      L.l0[j] = sg * (int)a0;
+      const double v =
+          zero ? 0.0
+               : __builtin_rint(__builtin_ldexp(st.v[row][kq + j], bits - e));
+      const double h = __builtin_floor(v * 0x1p-32);
+      in.lo[j] = (uint32_t)__builtin_fma(-h, 0x1p32, v);
+      in.hi[j] = (uint32_t)(int)h + (1u << 30);
     } else {
-      L.l0[j] = (int)v;
+      const float v =
+          zero ? 0.0f
+               : __builtin_rintf(__builtin_ldexpf(st.v[row][kq + j], bits - e));
+      in.lo[j] = (uint32_t)(int)v + (1u << 30);
     }
   }
-  write_residues16<NMOD, WIDE>(L, P + (int64_t)r * NMOD * Kp + k0 + kq, Kp);
+  write_residues16<NMOD, WIDE>(in, P + (int64_t)r * NMOD * Kp + k0 + kq, Kp);
 }
 
 struct CrtParams {
