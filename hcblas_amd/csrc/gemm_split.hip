@@ -718,39 +718,93 @@ struct CrtParams {
   int nmod;
   const int8_t *Ap, *Bp;  // residue planes (rows of nmod * Kp)
   uint8_t *R;             // R[t][n][m]: (A'B')(m, n) mod m_t in [0, m_t)
-  int tiles_r, tiles_c;
+  int tiles_m, tiles_n;
 };
 
-// One INT8 GEMM per modulus (blockIdx.z = t). The tile is computed as
-// (A'B')^T so that the MFMA output lanes run along m, the contiguous
-// dimension of R.
+// Residues of one BM x BN tile of A'B' (rows along m, columns along n)
+// modulo `mod`, written to R (n-major, m contiguous) through LDS. In the
+// 32x32 MFMA output layout a lane's values 4g..4g+3 of a block are four
+// consecutive rows (m), so they pack into one dword of the LDS tile
+// [n][m] (row stride BM + 16 bytes); the tile then leaves with 16-byte
+// stores, a row of BM bytes per 16 lanes. Storing every value as its own
+// byte straight from the accumulator layout costs one global store
+// instruction per value (128 per lane for 256 x 256 tiles).
+// a mod m: |a| <= K * 128^2 <= 2^30, so q = floor(float(a) / m) is within
+// one of the true quotient and |q| < 2^23 (24-bit multiply).
+template <int BM, int BN, int WM, int WN, int LDS_BYTES, typename AccArr>
+__device__ __forceinline__ void crt_store_tile(const AccArr &acc, int mod,
+                                               uint8_t *R, int M, int N,
+                                               int m0, int n0, char *lds) {
+  constexpr int MI = 32;
+  using WT = WaveTile<Elem::I8, BM, BN, WM, WN, MI>;
+  constexpr int LROW = BM + 16;
+  static_assert(BN * LROW <= LDS_BYTES,
+                "residue tile fits the LDS of the main loop");
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wm = wave / WN, wn = wave % WN;
+  const int lrow = lane & 31, lgrp = lane >> 5;
+  const float inv = 1.0f / (float)mod;
+  auto red = [&](int a) {
+    const int q = (int)__builtin_floorf((float)a * inv);
+    int u = a - __mul24(q, mod);
+    u += u < 0 ? mod : 0;
+    u -= u >= mod ? mod : 0;
+    return u;
+  };
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < WT::FM; ++i)
+#pragma unroll
+    for (int j = 0; j < WT::FN; ++j) {
+      const int n = wn * WT::WTN + j * MI + lrow;
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const int m = wm * WT::WTM + i * MI + 8 * g + 4 * lgrp;
+        const uint32_t w =
+            pack_bytes(red(acc[i][j][4 * g]), red(acc[i][j][4 * g + 1]),
+                       red(acc[i][j][4 * g + 2]), red(acc[i][j][4 * g + 3]));
+        *reinterpret_cast<uint32_t *>(lds + n * LROW + m) = w;
+      }
+    }
+  __syncthreads();
+  constexpr int CH = BM / 16;  // 16-byte chunks per row
+  constexpr int NTH = 64 * WM * WN;
+  const bool vec = (M & 15) == 0;
+#pragma unroll
+  for (int c = threadIdx.x; c < BN * CH; c += NTH) {
+    const int nl = c / CH, ml = (c % CH) * 16;
+    const int n = n0 + nl, m = m0 + ml;
+    if (n >= N || m >= M) continue;
+    uint8_t *dst = R + (int64_t)n * M + m;
+    const char *src = lds + nl * LROW + ml;
+    if (vec) {
+      *reinterpret_cast<vec4u *>(dst) = *reinterpret_cast<const vec4u *>(src);
+    } else {
+      const int e = M - m < 16 ? M - m : 16;
+      for (int b = 0; b < e; ++b) dst[b] = (uint8_t)src[b];
+    }
+  }
+}
+
+// One INT8 GEMM per modulus (blockIdx.z = t) over the residue planes, rows
+// of the tile along m (A planes), columns along n (B planes).
 template <int BM, int BN, int BK, int WM, int WN, int MI>
 __global__ __launch_bounds__(64 * WM * WN) void crt_gemm_kernel(CrtParams p) {
-  __shared__ __attribute__((aligned(16)))
-  char lds[tile_lds_bytes<Elem::I8, BM, BN, BK>()];
+  constexpr int LDS_BYTES = tile_lds_bytes<Elem::I8, BM, BN, BK>();
+  __shared__ __attribute__((aligned(16))) char lds[LDS_BYTES];
   using WT = WaveTile<Elem::I8, BM, BN, WM, WN, MI>;
   const int t = blockIdx.z;
   const int mod = d_moduli[t];
-  int tr, tc;
-  tile_coords(blockIdx.x, p.tiles_r, p.tiles_c, tr, tc);
+  int tm, tn;
+  tile_coords(blockIdx.x, p.tiles_m, p.tiles_n, tm, tn);
   const int64_t ld = (int64_t)p.nmod * p.Kp;
   typename WT::AccV acc[WT::FM][WT::FN];
   gemm_mainloop<Elem::I8, true, true, BM, BN, BK, WM, WN, true, true, MI,
-                true>(p.Bp + t * p.Kp, ld, p.Ap + t * p.Kp, ld, p.N, p.M,
-                      (int)p.Kp, tr * BM, tc * BN, lds, acc);
-  uint8_t *R = p.R + (int64_t)t * p.M * p.N;
-  // a mod m without an integer division: |a| <= Kp * 2^14 <= 2^30, so
-  // float(a) * (1/m) is within 1 of a/m (float(a) is off by at most 32,
-  // 1/m by a relative 2^-24), and one correction each way is enough.
-  const float inv = 1.0f / (float)mod;
-  for_each_acc<Elem::I8, BM, BN, WM, WN, MI>(
-      acc, p.N, p.M, tr * BM, tc * BN, [&](int n, int m, int a) {
-        const int q = (int)__builtin_floorf((float)a * inv);
-        int u = a - q * mod;
-        u += u < 0 ? mod : 0;
-        u -= u >= mod ? mod : 0;
-        R[(int64_t)n * p.M + m] = (uint8_t)u;
-      });
+                true>(p.Ap + t * p.Kp, ld, p.Bp + t * p.Kp, ld, p.M, p.N,
+                      (int)p.Kp, tm * BM, tn * BN, lds, acc);
+  crt_store_tile<BM, BN, WM, WN, LDS_BYTES>(
+      acc, mod, p.R + (int64_t)t * p.M * p.N, p.M, p.N, tm * BM, tn * BN,
+      lds);
 }
 
 // Reconstruction by the explicit CRT. With P = prod m_t, M_t = P / m_t and
@@ -1235,10 +1289,10 @@ hcblasStatus_t gemm_ozaki_crt(hcblasHandle_t h, const Canon &c, double alpha,
       p.Ap = Ap;
       p.Bp = Bp;
       p.R = R;
-      p.tiles_r = (int)cdiv(c.N, BM);
-      p.tiles_c = (int)cdiv(c.M, BN);
+      p.tiles_m = (int)cdiv(c.M, BM);
+      p.tiles_n = (int)cdiv(c.N, BN);
       crt_gemm_kernel<BM, BN, BK, WM, WN, MI>
-          <<<dim3((unsigned)(p.tiles_r * p.tiles_c), 1, (unsigned)nmod),
+          <<<dim3((unsigned)(p.tiles_m * p.tiles_n), 1, (unsigned)nmod),
              64 * WM * WN, 0, h->stream>>>(p);
       st = launch_status();
       if (st != HCBLAS_STATUS_SUCCESS) return st;
@@ -1345,10 +1399,10 @@ hcblasStatus_t gemm_ozaki_crt_f32(hcblasHandle_t h, const Canon &c,
       p.Ap = Ap;
       p.Bp = Bp;
       p.R = R;
-      p.tiles_r = (int)cdiv(c.N, BM);
-      p.tiles_c = (int)cdiv(c.M, BN);
+      p.tiles_m = (int)cdiv(c.M, BM);
+      p.tiles_n = (int)cdiv(c.N, BN);
       crt_gemm_kernel<BM, BN, BK, WM, WN, MI>
-          <<<dim3((unsigned)(p.tiles_r * p.tiles_c), 1, (unsigned)nmod),
+          <<<dim3((unsigned)(p.tiles_m * p.tiles_n), 1, (unsigned)nmod),
              64 * WM * WN, 0, h->stream>>>(p);
       st = launch_status();
       if (st != HCBLAS_STATUS_SUCCESS) return st;
