@@ -77,51 +77,66 @@ struct TileStage {
   }
 };
 
-// 64 (r) x 64 (k) tile of an R x K operand view staged through LDS (row
-// stride 65 elements), read along the source's contiguous dimension. With
-// vec (the base pointer and ld are multiples of the 16-byte vector) every
-// lane reads whole 16-byte vectors; partial vectors at the edges and
+// RT (r) x KT (k) tile of an R x K operand view staged through LDS (row
+// stride KT + 1 elements), read along the source's contiguous dimension.
+// With vec (the base pointer and ld are multiples of the 16-byte vector)
+// every lane reads whole 16-byte vectors; partial vectors at the edges and
 // unaligned operands are read element by element, with zeros outside the
 // view.
-template <typename T>
-struct Tile64 {
-  T v[64][65];
-  __device__ __forceinline__ void load(const T *__restrict__ X, int64_t ld,
-                                       bool kcont, bool vec, int R, int K,
-                                       int r0, int k0) {
+template <typename T, int RT, int KT>
+struct TileRK {
+  T v[RT][KT + 1];
+  template <bool KC>
+  __device__ __forceinline__ void load_as(const T *__restrict__ X, int64_t ld,
+                                          bool vec, int R, int K, int r0,
+                                          int k0) {
     constexpr int VE = 16 / (int)sizeof(T);  // elements per vector
-    constexpr int LPL = 64 / VE;             // lanes per 64-element line
+    constexpr int LEN = KC ? KT : RT;        // elements per source line
+    constexpr int NL = KC ? RT : KT;         // source lines in the tile
+    constexpr int LPL = LEN / VE;            // lanes per line
     constexpr int LPP = 256 / LPL;           // lines per pass
+    static_assert(LPL * VE == LEN && LPP * LPL == 256 && NL % LPP == 0,
+                  "tile shape");
     const int line0 = threadIdx.x / LPL, c = (threadIdx.x % LPL) * VE;
 #pragma unroll
-    for (int p = 0; p < 64 / LPP; ++p) {
+    for (int p = 0; p < NL / LPP; ++p) {
       const int line = line0 + p * LPP;
-      const int r = kcont ? r0 + line : r0 + c;
-      const int k = kcont ? k0 + c : k0 + line;
-      const T *src = kcont ? X + (int64_t)r * ld + k : X + (int64_t)k * ld + r;
+      const int r = KC ? r0 + line : r0 + c;
+      const int k = KC ? k0 + c : k0 + line;
+      const T *src = KC ? X + (int64_t)r * ld + k : X + (int64_t)k * ld + r;
       T t[VE];
-      const bool full = kcont ? (r < R && k + VE <= K) : (k < K && r + VE <= R);
+      const bool full = KC ? (r < R && k + VE <= K) : (k < K && r + VE <= R);
       if (vec && full) {
         const vec4u w = *reinterpret_cast<const vec4u *>(src);
         __builtin_memcpy(t, &w, 16);
       } else {
 #pragma unroll
         for (int e = 0; e < VE; ++e) {
-          const bool in = kcont ? (r < R && k + e < K) : (k < K && r + e < R);
+          const bool in = KC ? (r < R && k + e < K) : (k < K && r + e < R);
           t[e] = in ? src[e] : T(0);
         }
       }
 #pragma unroll
       for (int e = 0; e < VE; ++e) {
-        if (kcont)
+        if (KC)
           v[line][c + e] = t[e];
         else
           v[c + e][line] = t[e];
       }
     }
+  }
+  __device__ __forceinline__ void load(const T *__restrict__ X, int64_t ld,
+                                       bool kcont, bool vec, int R, int K,
+                                       int r0, int k0) {
+    if (kcont)
+      load_as<true>(X, ld, vec, R, K, r0, k0);
+    else
+      load_as<false>(X, ld, vec, R, K, r0, k0);
     __syncthreads();
   }
 };
+template <typename T>
+using Tile64 = TileRK<T, 64, 64>;
 
 // True when 16-byte vector reads of an operand with leading dimension ld are
 // aligned (the base pointer and every line start).
@@ -652,12 +667,22 @@ __device__ __forceinline__ void write_residues16(const ResidueIn<WIDE> &in,
         q[j] = (int)in.lo[j];  // v and lo agree modulo 2^8
       } else {
         using W = ResidueWeights<M, WIDE>;
-        uint32_t acc = W::k;
+        // The accumulator starts at the bits of 2^23, so it ends as the
+        // float 2^23 + s (s < 2^20) without a conversion.
+        uint32_t acc = W::k + 0x4B000000u;
         if constexpr (WIDE) acc = __builtin_amdgcn_udot4(in.hi[j], W::chi, acc, false);
         acc = __builtin_amdgcn_udot4(in.lo[j], W::clo, acc, false);
         constexpr float inv = 1.0f / (float)M;
-        const float sf = (float)acc;
-        q[j] = (int)__builtin_fmaf(-__builtin_rintf(sf * inv), (float)M, sf);
+        constexpr float c23 = 8388608.0f * inv;  // 2^23 / m as rounded: exact
+        // (2^23 + s) * inv - 2^23 * inv is s * inv exactly before the one
+        // rounding, so the quotient is rint(fl(s / m)) as above.
+        const float qf = __builtin_rintf(
+            __builtin_fmaf(__uint_as_float(acc), inv, -c23));
+        // 1.5 * 2^23 + s - q m = 1.5 * 2^23 + r is an exact float whose low
+        // mantissa byte is r's two's-complement byte (|r| < 2^22).
+        const float rf = __builtin_fmaf(-qf, (float)M,
+                                        __uint_as_float(acc | 0x400000u));
+        q[j] = (int)__float_as_uint(rf);
       }
     }
     vec4u w;
@@ -671,10 +696,11 @@ __device__ __forceinline__ void write_residues16(const ResidueIn<WIDE> &in,
 }
 
 // Residue planes P[r][t][0..Kp) of v = rint(X(r, k) * 2^(bits - e_r)) (Kp a
-// multiple of 64; zeros past K). A workgroup stages a 64 x 64 tile through
+// multiple of 128; zeros past K). A workgroup stages a 32 x 128 tile through
 // LDS, reading along the source's contiguous dimension; each thread then
 // converts 16 consecutive k of one row and writes one 16-byte vector per
-// plane. FP32 input: bits - e_r keeps |v| <= 2^30, and the scaling and
+// plane, so eight threads write a whole 128-byte line of each plane row
+// (a 64-wide tile wrote half lines, which the memory merges less well). FP32 input: bits - e_r keeps |v| <= 2^30, and the scaling and
 // rounding are exact in float (values below 2^-126 round to 0 either way);
 // FP64 input: |v| <= 2^62, split exactly into 32-bit halves in double.
 template <typename T, int NMOD>
@@ -683,11 +709,11 @@ __global__ __launch_bounds__(256) void crt_residue_kernel(
     const int *__restrict__ exps, int bits, int8_t *__restrict__ P,
     int64_t Kp) {
   constexpr bool WIDE = sizeof(T) == 8;
-  __shared__ Tile64<T> st;
-  const int r0 = blockIdx.y * 64, k0 = blockIdx.x * 64;
+  __shared__ TileRK<T, 32, 128> st;
+  const int r0 = blockIdx.y * 32, k0 = blockIdx.x * 128;
   const int tid = threadIdx.x;
   st.load(X, ld, kcont, vec, R, K, r0, k0);
-  const int row = tid >> 2, kq = (tid & 3) * 16;
+  const int row = tid >> 3, kq = (tid & 7) * 16;
   const int r = r0 + row;
   if (r >= R) return;
   const int e = exps[r];
@@ -912,6 +938,11 @@ __device__ __forceinline__ double crt_value(const int *u) {
 #pragma unroll
   for (int j = 0; j < NP; ++j) qf = __builtin_fma(S[j], c.f[j], qf);
   const double q = __builtin_rint(qf);
+  if constexpr (NP == 2) {
+    // D_1 2^32 + D_0 with exact D_j: one rounding gives C correctly rounded.
+    return __builtin_fma(__builtin_fma(-q, c.p[1], S[1]), 4294967296.0,
+                         __builtin_fma(-q, c.p[0], S[0]));
+  }
   double hi = __builtin_fma(-q, c.p[NP - 1], S[NP - 1]), lo = 0.0;
 #pragma unroll
   for (int j = NP - 2; j >= 0; --j) {
@@ -1005,7 +1036,8 @@ template <int NMOD, typename T>
 void launch_crt_residues(hipStream_t s, int R, int K, const T *X, int64_t ld,
                          bool kcont, const int *exps, int bits, int8_t *P,
                          int64_t Kp) {
-  dim3 g((unsigned)cdiv(Kp, 64), (unsigned)cdiv(R, 64));
+  // Kp is a multiple of 128 (the INT8 GEMM's K step).
+  dim3 g((unsigned)(Kp / 128), (unsigned)cdiv(R, 32));
   crt_residue_kernel<T, NMOD><<<g, 256, 0, s>>>(
       R, K, X, ld, kcont, vec_aligned(X, ld), exps, bits, P, Kp);
 }
