@@ -312,3 +312,61 @@ def test_bf16_strided_batched_fp32_out(hc, handle, torch_cuda):
     ref = torch.bmm(A.double(), B.double().contiguous())
     err = (C.double() - ref).abs().max().item()
     assert err <= K * 2 ** -23 * 4, err
+
+
+# Sizes of the reference's generators (test/unit/test_constants.h:18-64,
+# glibc srand(1)): vsmall = 48, large = 8378, vlarge = 14522. Shapes of
+# test/unit/hcblas_sgemm.cpp: slim A (M = N = vlarge, K = vsmall,
+# :1190-1313), slim C (M = K = vlarge, N = vsmall, :1315-1437) and the large
+# square sizes of :962-1188, column-major, all four trans cases. The
+# comparison is the reference's relative L2 error over sampled entries
+# (test/include/helper_functions.h:10-53, 1e-5 in FP32) against the
+# double-double value, plus the element-wise c*K*eps bound.
+def _sampled_check(hc, handle, orc, torch, p, ta, tb, M, N, K, seed):
+    api = hc.api
+    dt = DT[p]
+    ar, ac = (M, K) if ta == N_ else (K, M)
+    br, bc = (K, N) if tb == N_ else (N, K)
+    A = _rand(orc, p, ar * ac, seed + 1)
+    B = _rand(orc, p, br * bc, seed + 2)
+    big_c = M * N > 50_000_000
+    alpha = 0.75
+    beta = 0.0 if big_c else -1.25
+    C = np.zeros(1, dt) if big_c else _rand(orc, p, M * N, seed + 3)
+    dC = (torch.empty(M * N, dtype=torch.float32 if p == "s" else torch.float64,
+                      device="cuda") if big_c else to_dev(C))
+    st = getattr(api, f"hcblas_{p}gemm")(handle, COL, ta, tb, M, N, K, alpha,
+                                         to_dev(A), ar, to_dev(B), br, beta,
+                                         dC, M, 0, 0, 0)
+    assert st == 0
+    rng = np.random.default_rng(seed)
+    ns = 512
+    rows = rng.integers(0, M, ns)
+    cols = rng.integers(0, N, ns)
+    idx = torch.as_tensor(rows + cols * M, device="cuda")
+    got = to_host(dC[idx]).astype(np.float64)
+    ref = orc.gemm_sample_exact(COL, ta, tb, M, N, K, alpha, A, ar, B, br,
+                                beta, C, M, rows, cols)
+    eps = np.finfo(dt).eps
+    rel_l2 = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    assert rel_l2 <= (1e-5 if p == "s" else 1e-13), rel_l2
+    tol = 2 * (K + 2) * eps * (abs(alpha) * K + abs(beta) + 1)
+    assert np.abs(got - ref).max() <= tol
+
+
+@pytest.mark.parametrize("p", ["s", "d"])
+@pytest.mark.parametrize("M,N,K", [(14522, 14522, 48), (14522, 48, 14522)])
+def test_reference_slim_shapes(hc, handle, orc, torch_cuda, p, M, N, K):
+    for ta in (N_, T_):
+        for tb in (N_, T_):
+            _sampled_check(hc, handle, orc, torch_cuda, p, ta, tb, M, N, K,
+                           seed=M + 3 * N + 7 * K + 11 * ta + 13 * tb)
+    torch_cuda.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("ta,tb", [(N_, N_), (N_, T_), (T_, N_), (T_, T_)])
+def test_reference_large_square(hc, handle, orc, torch_cuda, ta, tb):
+    n = 8378
+    _sampled_check(hc, handle, orc, torch_cuda, "s", ta, tb, n, n, n,
+                   seed=n + 11 * ta + 13 * tb)
+    torch_cuda.cuda.empty_cache()

@@ -76,7 +76,8 @@ def _rel_errors(orc, o, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
 
 
 @pytest.mark.parametrize("p", ["s", "sc", "d", "dc"])
-@pytest.mark.parametrize("M,N,K", [(33, 17, 45), (128, 256, 512), (300, 129, 1000)])
+@pytest.mark.parametrize("M,N,K", [(33, 17, 45), (128, 256, 512), (300, 129, 1000),
+                                   (272, 300, 130)])
 def test_all_cases_within_plain_bound(hc, handle, orc, p, M, N, K):
     eps = np.finfo(np.float32 if p[0] == "s" else np.float64).eps
     for o in (COL, ROW):
