@@ -19,9 +19,10 @@
 // static schedule) candidate on the plan's own problems and keeps the
 // fastest.
 //
-// Each tile runs gemm_tile, the same device code as the plain GEMM with the
-// same K step, so a grouped result is bit-identical to per-problem
-// hcblas_sgemm / hcblas_hgemm launches.
+// Each tile runs the plain GEMM's main loop with the same K step and an
+// epilogue with the same arithmetic (staged through LDS so that C is written
+// with 16-byte column runs), so a grouped result is bit-identical to
+// per-problem hcblas_sgemm / hcblas_hgemm launches.
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -35,7 +36,8 @@ namespace {
 
 struct ProbDev {
   int dtype;  // 0: fp32, 1: fp16
-  int flags;  // bit 0: A' k-contiguous, bit 1: B' k-contiguous
+  int flags;  // bit 0: A' k-contiguous, bit 1: B' k-contiguous, bit 2: C
+              // allows 16-byte vector accesses
   int M, N, K;
   int tiles_n;
   float alpha, beta;
@@ -60,34 +62,38 @@ struct GroupCfg {
           : tile_lds_bytes<Elem::F16, TM, TM, kF16BK>();
 };
 
+// One tile: the plain kernels' main loop (same K step, so the sums are
+// bit-identical to per-problem launches), then the LDS-staged epilogue.
+template <Elem E, typename OutT, int TM, bool VL, bool A_K, bool B_K>
+__device__ __forceinline__ void tile_body(const ProbDev &p, int m0, int n0,
+                                          char *lds) {
+  using T = typename ElemT<E>::T;
+  constexpr int BK = E == Elem::F32 ? kF32BK : kF16BK;
+  using WT = WaveTile<E, TM, TM, 2, 2, 16>;
+  typename WT::AccV acc[WT::FM][WT::FN];
+  gemm_mainloop<E, A_K, B_K, TM, TM, BK, 2, 2, VL, VL, 16>(
+      static_cast<const T *>(p.A), p.lda, static_cast<const T *>(p.B), p.ldb,
+      p.M, p.N, p.K, m0, n0, lds, acc);
+  store_tile_lds<E, OutT, TM, TM, 2, 2>(acc, static_cast<OutT *>(p.C), p.ldc,
+                                        p.M, p.N, m0, n0, p.alpha, p.beta,
+                                        lds, (p.flags & 4) != 0);
+}
+
 template <Elem E, typename OutT, int TM, bool VL>
 __device__ __forceinline__ void run_tile(const ProbDev &p, int m0, int n0,
                                          char *lds) {
-  using T = typename ElemT<E>::T;
-  constexpr int BK = E == Elem::F32 ? kF32BK : kF16BK;
-  const T *A = static_cast<const T *>(p.A);
-  const T *B = static_cast<const T *>(p.B);
-  OutT *C = static_cast<OutT *>(p.C);
   switch (p.flags & 3) {
     case 0:
-      gemm_tile<E, OutT, false, false, TM, TM, BK, 2, 2, VL, VL>(
-          A, p.lda, B, p.ldb, C, p.ldc, p.M, p.N, p.K, p.alpha, p.beta, m0,
-          n0, lds);
+      tile_body<E, OutT, TM, VL, false, false>(p, m0, n0, lds);
       break;
     case 1:
-      gemm_tile<E, OutT, true, false, TM, TM, BK, 2, 2, VL, VL>(
-          A, p.lda, B, p.ldb, C, p.ldc, p.M, p.N, p.K, p.alpha, p.beta, m0,
-          n0, lds);
+      tile_body<E, OutT, TM, VL, true, false>(p, m0, n0, lds);
       break;
     case 2:
-      gemm_tile<E, OutT, false, true, TM, TM, BK, 2, 2, VL, VL>(
-          A, p.lda, B, p.ldb, C, p.ldc, p.M, p.N, p.K, p.alpha, p.beta, m0,
-          n0, lds);
+      tile_body<E, OutT, TM, VL, false, true>(p, m0, n0, lds);
       break;
     default:
-      gemm_tile<E, OutT, true, true, TM, TM, BK, 2, 2, VL, VL>(
-          A, p.lda, B, p.ldb, C, p.ldc, p.M, p.N, p.K, p.alpha, p.beta, m0,
-          n0, lds);
+      tile_body<E, OutT, TM, VL, true, true>(p, m0, n0, lds);
       break;
   }
 }
@@ -346,6 +352,8 @@ hcblasStatus_t hcblasGroupedPlanCreate(hcblasHandle_t handle,
       return ptr == nullptr ||
              (reinterpret_cast<uintptr_t>(ptr) % 16 == 0);
     };
+    // bit 2: 16-byte vector accesses of C in the epilogue
+    if (q.ldc % vec == 0 && al(q.C)) q.flags |= 4;
     (void)es;
     if (q.lda % vec || q.ldb % vec || !al(q.A) || !al(q.B)) vload = false;
     // r-contiguous operands: whole vectors of rows (see TileLoader)

@@ -564,6 +564,84 @@ __device__ __forceinline__ void gemm_tile(
       });
 }
 
+// Epilogue of a BM x BN tile through LDS, for tiles whose K loop is short
+// (grouped GEMM): there the stores weigh as much as the main loop, and the
+// direct form above stores one value per lane per instruction across 16
+// columns. The accumulators (16x16 blocks: a lane's values r = 0..3 are
+// four consecutive rows) go to LDS as [n][m] with 16-byte writes (row
+// stride BM + 4 floats: the 16 lanes of a column group hit every bank once),
+// then each thread blends and stores VE consecutive rows of one column with
+// one vector access (VE = 16 bytes of the output type), a column's BM rows
+// by BM / VE consecutive lanes. The arithmetic is the direct form's
+// (alpha * acc, then fma(beta, C, .)), so results are bit-identical.
+// vec_c: C and ldc allow VE-element vector accesses. Ends with a barrier,
+// so the caller may reuse the LDS at once.
+template <Elem E, typename OutT, int BM, int BN, int WM, int WN, typename AccArr>
+__device__ __forceinline__ void store_tile_lds(const AccArr &acc, OutT *C,
+                                               int64_t ldc, int M, int N,
+                                               int m0, int n0, double alpha_d,
+                                               double beta_d, char *lds,
+                                               bool vec_c) {
+  constexpr int MI = 16;
+  using WT = WaveTile<E, BM, BN, WM, WN, MI>;
+  using Acc = typename ElemT<E>::Acc;
+  static_assert(sizeof(Acc) == 4, "FP32 accumulators");
+  constexpr int LROW = BM + 4;  // floats
+  constexpr int VE = 16 / (int)sizeof(OutT);
+  constexpr int NTH = 64 * WM * WN;
+  static_assert(BM % VE == 0 && NTH % (BM / VE) == 0, "epilogue shape");
+  float *t = reinterpret_cast<float *>(lds);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wm = wave / WN, wn = wave % WN;
+  const int lrow = lane % MI, lgrp = lane / MI;
+#pragma unroll
+  for (int i = 0; i < WT::FM; ++i)
+#pragma unroll
+    for (int j = 0; j < WT::FN; ++j) {
+      const int n = wn * WT::WTN + j * MI + lrow;
+      const int m = wm * WT::WTM + i * MI + 4 * lgrp;
+      *reinterpret_cast<floatx4 *>(t + n * LROW + m) = acc[i][j];
+    }
+  __syncthreads();
+  const Acc alpha = (Acc)alpha_d, beta = (Acc)beta_d;
+  constexpr int TPC = BM / VE;  // threads per column
+#pragma unroll
+  for (int c0 = 0; c0 < BN; c0 += NTH / TPC) {
+    const int nl = c0 + (int)threadIdx.x / TPC;
+    const int ml = ((int)threadIdx.x % TPC) * VE;
+    const int n = n0 + nl, m = m0 + ml;
+    if (nl < BN && n < N && m < M) {
+      OutT *dst = C + (int64_t)n * ldc + m;
+      const float *src = t + nl * LROW + ml;
+      OutT o[VE];
+      if (vec_c && m + VE <= M) {
+        OutT cin[VE];
+        if (beta != Acc(0)) {
+          const vec4u w = *reinterpret_cast<const vec4u *>(dst);
+          __builtin_memcpy(cin, &w, 16);
+        }
+#pragma unroll
+        for (int e = 0; e < VE; ++e) {
+          Acc v = alpha * src[e];
+          if (beta != Acc(0)) v = fma_acc(beta, from_out<Acc>(cin[e]), v);
+          o[e] = to_out<OutT>(v);
+        }
+        vec4u w;
+        __builtin_memcpy(&w, o, 16);
+        *reinterpret_cast<vec4u *>(dst) = w;
+      } else {
+        const int ne = M - m < VE ? M - m : VE;
+        for (int e = 0; e < ne; ++e) {
+          Acc v = alpha * src[e];
+          if (beta != Acc(0)) v = fma_acc(beta, from_out<Acc>(dst[e]), v);
+          dst[e] = to_out<OutT>(v);
+        }
+      }
+    }
+  }
+  __syncthreads();
+}
+
 template <Elem E, typename OutT, bool A_K, bool B_K, int BM, int BN, int BK,
           int WM, int WN, bool VA, bool VB, int MI = 16>
 __global__ __launch_bounds__(64 * WM * WN) void gemm_mfma_kernel(
