@@ -1,7 +1,7 @@
 """Shared test configuration.
 
 Markers:
-  gpu -- needs a real MI355X (gfx950); runs on the GPU box only.
+  gpu -- needs a real MI355X (gfx950).
 
 CPU tests (``-m "not gpu"``) cover the oracle against the golden fixtures,
 the host-side argument logic and that the C-ABI library loads and exports
