@@ -125,3 +125,88 @@ def test_grouped_rejects_invalid(hc, handle, torch_cuda):
     assert lib.hcblasGroupedPlanCreate(handle.value, None, 0, ctypes.byref(p)) == 0
     assert lib.hcblasGroupedGemm(handle.value, p) == 0
     assert lib.hcblasGroupedPlanDestroy(p) == 0
+
+
+def _irregular_table(torch, hc, seed, aligned):
+    """Problems with odd sizes, both transposes, padded leading dimensions,
+    alpha != 1 and beta != 0 (C is read), in both orders and dtypes. aligned:
+    leading dimensions rounded up to multiples of 8 and 256-byte aligned
+    matrices; otherwise odd padding and matrices at odd element offsets."""
+    rng = np.random.default_rng(seed)
+    sizes = [1, 5, 8, 17, 33, 64, 70, 130]
+    meta, used = [], [0, 0]
+    for i in range(64):
+        d = int(rng.integers(0, 2))
+        o = int(rng.integers(0, 2))
+        ta, tb = int(rng.integers(0, 2)), int(rng.integers(0, 2))
+        m, n, k = (int(rng.choice(sizes)) for _ in range(3))
+        alpha = float(rng.choice([1.0, -0.75]))
+        beta = float(rng.choice([0.0, 0.5]))
+        col = o == 1
+
+        def stored(r, c):
+            ld = r if col else c
+            # aligned: ld a multiple of 8 (vector accesses of C with a
+            # partial last vector when the size is odd); else odd padding
+            ld = (ld + 7) // 8 * 8 if aligned else ld + int(rng.choice([0, 1, 3]))
+            return ld, ld * (c if col else r)
+
+        lda, na = stored(m, k) if ta == 0 else stored(k, m)
+        ldb, nb = stored(k, n) if tb == 0 else stored(n, k)
+        ldc, nc = stored(m, n)
+        offs = []
+        for n_el in (na, nb, nc):
+            skew = int(rng.integers(0, 3)) if not aligned else 0
+            offs.append(used[d] + skew)
+            used[d] += (n_el + skew + 63) // 64 * 64
+        meta.append((d, o, ta, tb, m, n, k, alpha, beta, offs, lda, ldb, ldc))
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    ar32 = torch.rand(used[0], generator=g, device="cuda") * 2 - 1
+    ar16 = (torch.rand(used[1], generator=g, device="cuda") * 2 - 1).half()
+    arenas = (ar32, ar16)
+    problems = []
+    for (d, o, ta, tb, m, n, k, alpha, beta, offs, lda, ldb, ldc) in meta:
+        base, es = arenas[d].data_ptr(), (4 if d == 0 else 2)
+        problems.append(hc.GemmProblem(
+            hc.R_32F if d == 0 else hc.R_16F, o, ta, tb, m, n, k, 0, alpha,
+            beta, base + offs[0] * es, lda, base + offs[1] * es, ldb,
+            base + offs[2] * es, ldc))
+    return problems, meta, arenas
+
+
+@pytest.mark.parametrize("aligned", [True, False])
+def test_grouped_irregular_problems_bitwise(hc, handle, torch_cuda, aligned):
+    """Odd sizes, transposes, beta != 0 and unaligned C take the epilogue's
+    partial-vector and element paths: still bit-identical to per-problem
+    plain launches."""
+    torch = torch_cuda
+    api = hc.api
+    problems, meta, arenas = _irregular_table(torch, hc, 11 + aligned, aligned)
+    init = [t.clone() for t in arenas]
+    plan = hc.GroupedPlan(handle, problems)
+    try:
+        for tune in (False, True):
+            for d in (0, 1):
+                arenas[d].copy_(init[d])
+            if tune:
+                assert plan.autotune(1) == 0
+                for d in (0, 1):
+                    arenas[d].copy_(init[d])
+            assert plan.run() == 0
+            torch.cuda.synchronize()
+            got = [t.clone() for t in arenas]
+            ref = [t.clone() for t in init]
+            for (d, o, ta, tb, m, n, k, alpha, beta, offs, lda, ldb, ldc) in meta:
+                es = 4 if d == 0 else 2
+                fn = api.hcblas_sgemm if d == 0 else api.hcblas_hgemm
+                st = fn(handle, o, ta, tb, m, n, k, alpha,
+                        ref[d].data_ptr() + offs[0] * es, lda,
+                        ref[d].data_ptr() + offs[1] * es, ldb, beta,
+                        ref[d].data_ptr() + offs[2] * es, ldc, 0, 0, 0)
+                assert st == 0
+            torch.cuda.synchronize()
+            for d in (0, 1):
+                iv = torch.int32 if d == 0 else torch.int16
+                assert torch.equal(got[d].view(iv), ref[d].view(iv)), (d, tune)
+    finally:
+        plan.destroy()
