@@ -74,9 +74,9 @@ __device__ __forceinline__ void tile_body(const ProbDev &p, int m0, int n0,
   gemm_mainloop<E, A_K, B_K, TM, TM, BK, 2, 2, VL, VL, 16>(
       static_cast<const T *>(p.A), p.lda, static_cast<const T *>(p.B), p.ldb,
       p.M, p.N, p.K, m0, n0, lds, acc);
-  store_tile_lds<E, OutT, TM, TM, 2, 2>(acc, static_cast<OutT *>(p.C), p.ldc,
-                                        p.M, p.N, m0, n0, p.alpha, p.beta,
-                                        lds, (p.flags & 4) != 0);
+  store_tile_lds<E, OutT, TM, TM, 2, 2, 16, GroupCfg<TM>::LDS>(
+      acc, static_cast<OutT *>(p.C), p.ldc, p.M, p.N, m0, n0, p.alpha,
+      p.beta, lds, (p.flags & 4) != 0);
 }
 
 template <Elem E, typename OutT, int TM, bool VL>

@@ -540,6 +540,116 @@ __device__ __forceinline__ void for_each_acc(const AccArr &acc, int M, int N,
   }
 }
 
+// Epilogue of a BM x BN tile through LDS. Stored straight from the
+// accumulator layout, one store instruction writes one value per lane into
+// 16 or 32 different columns, so the tile leaves in BM*BN/64 scattered store
+// instructions; for short K loops (batched and grouped problems) that weighs
+// as much as the main loop. Here the accumulators go to LDS as [n][m] (m
+// contiguous, row stride BM plus 16 bytes), NC columns at a time (as many as
+// the main loop's LDS holds), and each thread then blends and stores VE
+// consecutive rows of one column with one 16-byte access, a column's BM rows
+// by BM / VE consecutive lanes. The arithmetic is the direct form's
+// (alpha * acc, then fma(beta, C, .)), so results are bit-identical.
+// Accumulator layouts (value r of lane l, block row / column):
+//   32x32 FP32/INT32: row 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32;
+//   16x16 FP32: row 4 (l / 16) + r, column l % 16;
+//   16x16 FP64: row l / 16 + 4 r, column l % 16.
+// vec_c: C and ldc allow 16-byte accesses. Expects the LDS free on entry
+// (the main loop ends with a barrier) and ends with a barrier.
+template <Elem E, typename OutT, int BM, int BN, int WM, int WN, int MI,
+          int LDS_BYTES, typename AccArr>
+__device__ __forceinline__ void store_tile_lds(const AccArr &acc, OutT *C,
+                                               int64_t ldc, int M, int N,
+                                               int m0, int n0, double alpha_d,
+                                               double beta_d, char *lds,
+                                               bool vec_c) {
+  using WT = WaveTile<E, BM, BN, WM, WN, MI>;
+  using Acc = typename ElemT<E>::Acc;
+  constexpr int LROW = BM + 16 / (int)sizeof(Acc);  // elements
+  constexpr int NC_FIT = LDS_BYTES / (LROW * (int)sizeof(Acc));
+  constexpr int NC = NC_FIT >= BN ? BN
+                     : NC_FIT >= 128 ? 128
+                     : NC_FIT >= 64  ? 64
+                     : NC_FIT >= 32  ? 32
+                                     : 16;
+  static_assert(NC >= MI && BN % NC == 0 && NC * LROW * (int)sizeof(Acc) <= LDS_BYTES,
+                "epilogue column chunk");
+  constexpr int VE = 16 / (int)sizeof(OutT);
+  constexpr int NTH = 64 * WM * WN;
+  constexpr int TPC = BM / VE;  // threads per column
+  static_assert(BM % VE == 0 && NTH % TPC == 0, "epilogue shape");
+  Acc *t = reinterpret_cast<Acc *>(lds);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wm = wave / WN, wn = wave % WN;
+  const int lrow = lane % MI, lgrp = lane / MI;
+  const Acc alpha = (Acc)alpha_d, beta = (Acc)beta_d;
+#pragma unroll
+  for (int c0 = 0; c0 < BN; c0 += NC) {
+#pragma unroll
+    for (int j = 0; j < WT::FN; ++j) {
+      const int cb = wn * WT::WTN + j * MI;  // wave-uniform
+      if (cb < c0 || cb >= c0 + NC) continue;
+      const int n = cb - c0 + lrow;
+#pragma unroll
+      for (int i = 0; i < WT::FM; ++i) {
+        const int mb = wm * WT::WTM + i * MI;
+        if constexpr (MI == 32) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const auto &a = acc[i][j];
+            floatx4 v = {(float)a[4 * q], (float)a[4 * q + 1],
+                         (float)a[4 * q + 2], (float)a[4 * q + 3]};
+            *reinterpret_cast<floatx4 *>(t + n * LROW + mb + 8 * q +
+                                         4 * lgrp) = v;
+          }
+        } else if constexpr (sizeof(Acc) == 8) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            t[n * LROW + mb + lgrp + 4 * r] = acc[i][j][r];
+        } else {
+          *reinterpret_cast<floatx4 *>(t + n * LROW + mb + 4 * lgrp) =
+              acc[i][j];
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int cc = 0; cc < NC; cc += NTH / TPC) {
+      const int nl = cc + (int)threadIdx.x / TPC;
+      const int ml = ((int)threadIdx.x % TPC) * VE;
+      const int n = n0 + c0 + nl, m = m0 + ml;
+      if (nl < NC && n < N && m < M) {
+        OutT *dst = C + (int64_t)n * ldc + m;
+        const Acc *src = t + nl * LROW + ml;
+        if (vec_c && m + VE <= M) {
+          OutT cin[VE], o[VE];
+          if (beta != Acc(0)) {
+            const vec4u w = *reinterpret_cast<const vec4u *>(dst);
+            __builtin_memcpy(cin, &w, 16);
+          }
+#pragma unroll
+          for (int e = 0; e < VE; ++e) {
+            Acc v = alpha * src[e];
+            if (beta != Acc(0)) v = fma_acc(beta, from_out<Acc>(cin[e]), v);
+            o[e] = to_out<OutT>(v);
+          }
+          vec4u w;
+          __builtin_memcpy(&w, o, 16);
+          *reinterpret_cast<vec4u *>(dst) = w;
+        } else {
+          const int ne = M - m < VE ? M - m : VE;
+          for (int e = 0; e < ne; ++e) {
+            Acc v = alpha * src[e];
+            if (beta != Acc(0)) v = fma_acc(beta, from_out<Acc>(dst[e]), v);
+            dst[e] = to_out<OutT>(v);
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // One BM x BN output tile at (m0, n0) of C = alpha*A'*B' + beta*C, the whole
 // K range; beta == 0 never reads C.
 template <Elem E, typename OutT, bool A_K, bool B_K, int BM, int BN, int BK,
@@ -549,97 +659,14 @@ __device__ __forceinline__ void gemm_tile(
     const typename ElemT<E>::T *__restrict__ B, int64_t ldb, OutT *C,
     int64_t ldc, int M, int N, int K, double alpha_d, double beta_d, int m0,
     int n0, char *lds) {
-  using Acc = typename ElemT<E>::Acc;
   using WT = WaveTile<E, BM, BN, WM, WN, MI>;
   typename WT::AccV acc[WT::FM][WT::FN];
   gemm_mainloop<E, A_K, B_K, BM, BN, BK, WM, WN, VA, VB, MI>(
       A, lda, B, ldb, M, N, K, m0, n0, lds, acc);
-  const Acc alpha = (Acc)alpha_d, beta = (Acc)beta_d;
-  for_each_acc<E, BM, BN, WM, WN, MI>(
-      acc, M, N, m0, n0, [&](int row, int col, auto a) {
-        OutT *dst = C + (int64_t)col * ldc + row;
-        Acc v = alpha * (Acc)a;
-        if (beta != Acc(0)) v = fma_acc(beta, from_out<Acc>(*dst), v);
-        *dst = to_out<OutT>(v);
-      });
-}
-
-// Epilogue of a BM x BN tile through LDS, for tiles whose K loop is short
-// (grouped GEMM): there the stores weigh as much as the main loop, and the
-// direct form above stores one value per lane per instruction across 16
-// columns. The accumulators (16x16 blocks: a lane's values r = 0..3 are
-// four consecutive rows) go to LDS as [n][m] with 16-byte writes (row
-// stride BM + 4 floats: the 16 lanes of a column group hit every bank once),
-// then each thread blends and stores VE consecutive rows of one column with
-// one vector access (VE = 16 bytes of the output type), a column's BM rows
-// by BM / VE consecutive lanes. The arithmetic is the direct form's
-// (alpha * acc, then fma(beta, C, .)), so results are bit-identical.
-// vec_c: C and ldc allow VE-element vector accesses. Ends with a barrier,
-// so the caller may reuse the LDS at once.
-template <Elem E, typename OutT, int BM, int BN, int WM, int WN, typename AccArr>
-__device__ __forceinline__ void store_tile_lds(const AccArr &acc, OutT *C,
-                                               int64_t ldc, int M, int N,
-                                               int m0, int n0, double alpha_d,
-                                               double beta_d, char *lds,
-                                               bool vec_c) {
-  constexpr int MI = 16;
-  using WT = WaveTile<E, BM, BN, WM, WN, MI>;
-  using Acc = typename ElemT<E>::Acc;
-  static_assert(sizeof(Acc) == 4, "FP32 accumulators");
-  constexpr int LROW = BM + 4;  // floats
   constexpr int VE = 16 / (int)sizeof(OutT);
-  constexpr int NTH = 64 * WM * WN;
-  static_assert(BM % VE == 0 && NTH % (BM / VE) == 0, "epilogue shape");
-  float *t = reinterpret_cast<float *>(lds);
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int wm = wave / WN, wn = wave % WN;
-  const int lrow = lane % MI, lgrp = lane / MI;
-#pragma unroll
-  for (int i = 0; i < WT::FM; ++i)
-#pragma unroll
-    for (int j = 0; j < WT::FN; ++j) {
-      const int n = wn * WT::WTN + j * MI + lrow;
-      const int m = wm * WT::WTM + i * MI + 4 * lgrp;
-      *reinterpret_cast<floatx4 *>(t + n * LROW + m) = acc[i][j];
-    }
-  __syncthreads();
-  const Acc alpha = (Acc)alpha_d, beta = (Acc)beta_d;
-  constexpr int TPC = BM / VE;  // threads per column
-#pragma unroll
-  for (int c0 = 0; c0 < BN; c0 += NTH / TPC) {
-    const int nl = c0 + (int)threadIdx.x / TPC;
-    const int ml = ((int)threadIdx.x % TPC) * VE;
-    const int n = n0 + nl, m = m0 + ml;
-    if (nl < BN && n < N && m < M) {
-      OutT *dst = C + (int64_t)n * ldc + m;
-      const float *src = t + nl * LROW + ml;
-      OutT o[VE];
-      if (vec_c && m + VE <= M) {
-        OutT cin[VE];
-        if (beta != Acc(0)) {
-          const vec4u w = *reinterpret_cast<const vec4u *>(dst);
-          __builtin_memcpy(cin, &w, 16);
-        }
-#pragma unroll
-        for (int e = 0; e < VE; ++e) {
-          Acc v = alpha * src[e];
-          if (beta != Acc(0)) v = fma_acc(beta, from_out<Acc>(cin[e]), v);
-          o[e] = to_out<OutT>(v);
-        }
-        vec4u w;
-        __builtin_memcpy(&w, o, 16);
-        *reinterpret_cast<vec4u *>(dst) = w;
-      } else {
-        const int ne = M - m < VE ? M - m : VE;
-        for (int e = 0; e < ne; ++e) {
-          Acc v = alpha * src[e];
-          if (beta != Acc(0)) v = fma_acc(beta, from_out<Acc>(dst[e]), v);
-          dst[e] = to_out<OutT>(v);
-        }
-      }
-    }
-  }
-  __syncthreads();
+  const bool vec_c = reinterpret_cast<uintptr_t>(C) % 16 == 0 && ldc % VE == 0;
+  store_tile_lds<E, OutT, BM, BN, WM, WN, MI, tile_lds_bytes<E, BM, BN, BK>()>(
+      acc, C, ldc, M, N, m0, n0, alpha_d, beta_d, lds, vec_c);
 }
 
 template <Elem E, typename OutT, bool A_K, bool B_K, int BM, int BN, int BK,
