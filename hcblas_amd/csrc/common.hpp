@@ -63,6 +63,8 @@ struct hcblasContext {
   hcb::Workspace split_ws;
   // FP32 copies of BF16 operands for the adjoint GEMMs.
   hcb::Workspace adj_ws;
+  // Split-K partial tiles of the plain GEMM.
+  hcb::Workspace kpart_ws;
 };
 
 namespace hcb {

@@ -334,6 +334,20 @@ hcblasStatus_t gemm_real(hcblasHandle_t h, const Canon &c, double alpha,
                         (batch == 1 || c.strideA % V == 0);
   if (!c.Bptrs) vload = vload && ptr_aligned(c.B, c.bOff, sizeof(T)) &&
                         (batch == 1 || c.strideB % V == 0);
+  if (batch == 1 && vload && !c.Aptrs && !c.Bptrs && !Cptrs) {
+    int ks = 0;
+    const int S = splitk_count(c.M, c.N, c.K, h->num_cus,
+                               E == Elem::F64 ? 16 : E == Elem::F32 ? 32 : 64,
+                               ks);
+    if (S > 1) {
+      const size_t bytes = sizeof(Acc) * (size_t)S * c.M * c.N;
+      if (ensure_workspace(h->kpart_ws, bytes) != hipSuccess)
+        return HCBLAS_STATUS_ALLOC_FAILED;
+      return launch_gemm_splitk<E, OutT>(h->stream, h->num_cus, p, c.a_k,
+                                         c.b_k, S, ks,
+                                         static_cast<Acc *>(h->kpart_ws.ptr));
+    }
+  }
   return launch_gemm<E, OutT>(h->stream, h->num_cus, p, c.a_k, c.b_k, vload,
                               batch);
 }

@@ -5,4 +5,7 @@ namespace hcb {
 template hcblasStatus_t launch_gemm<Elem::F16, uint16_t>(hipStream_t, int,
                                                      GemmParams, bool, bool,
                                                      bool, int64_t);
+template hcblasStatus_t launch_gemm_splitk<Elem::F16, uint16_t>(
+    hipStream_t, int, GemmParams, bool, bool, int, int,
+    typename ElemT<Elem::F16>::Acc *);
 }  // namespace hcb

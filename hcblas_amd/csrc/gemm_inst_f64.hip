@@ -5,4 +5,7 @@ namespace hcb {
 template hcblasStatus_t launch_gemm<Elem::F64, double>(hipStream_t, int,
                                                      GemmParams, bool, bool,
                                                      bool, int64_t);
+template hcblasStatus_t launch_gemm_splitk<Elem::F64, double>(
+    hipStream_t, int, GemmParams, bool, bool, int, int,
+    typename ElemT<Elem::F64>::Acc *);
 }  // namespace hcb

@@ -698,4 +698,54 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_mfma_kernel(
       tile_m * BM, tile_n * BN, lds);
 }
 
+// Split-K partial tile: the K range [z ks, min(K, (z + 1) ks)) of one
+// tile of a single (unbatched) problem, stored without alpha / beta into
+// W + z M N (column-major, ld M). For problems with about as many tiles as
+// CUs and a long K, splitting K gives every CU several workgroups to switch
+// between; splitk_reduce_kernel then adds the partials in split order.
+template <Elem E, bool A_K, bool B_K, int BM, int BN, int BK, int WM, int WN,
+          bool VA, bool VB, int MI>
+__global__ __launch_bounds__(64 * WM * WN) void gemm_splitk_kernel(
+    GemmParams p, int ks, typename ElemT<E>::Acc *W) {
+  using T = typename ElemT<E>::T;
+  using Acc = typename ElemT<E>::Acc;
+  constexpr int LDS_BYTES = tile_lds_bytes<E, BM, BN, BK>();
+  __shared__ __attribute__((aligned(16))) char lds[LDS_BYTES];
+  const int z = blockIdx.z;
+  const int k0 = z * ks;
+  const int kl = p.K - k0 < ks ? p.K - k0 : ks;
+  const T *A = static_cast<const T *>(p.A) + p.aOff +
+               (A_K ? (int64_t)k0 : (int64_t)k0 * p.lda);
+  const T *B = static_cast<const T *>(p.B) + p.bOff +
+               (B_K ? (int64_t)k0 : (int64_t)k0 * p.ldb);
+  int tile_m, tile_n;
+  tile_coords(blockIdx.x, p.tiles_m, p.tiles_n, tile_m, tile_n);
+  using WT = WaveTile<E, BM, BN, WM, WN, MI>;
+  typename WT::AccV acc[WT::FM][WT::FN];
+  gemm_mainloop<E, A_K, B_K, BM, BN, BK, WM, WN, VA, VB, MI>(
+      A, p.lda, B, p.ldb, p.M, p.N, kl, tile_m * BM, tile_n * BN, lds, acc);
+  constexpr int VE = 16 / (int)sizeof(Acc);
+  store_tile_lds<E, Acc, BM, BN, WM, WN, MI, LDS_BYTES>(
+      acc, W + (int64_t)z * p.M * p.N, p.M, p.M, p.N, tile_m * BM,
+      tile_n * BN, 1.0, 0.0, lds, p.M % VE == 0);
+}
+
+// C = alpha * sum_z W[z] + beta * C, the partials added in split order.
+template <typename Acc, typename OutT>
+__global__ __launch_bounds__(256) void splitk_reduce_kernel(
+    int S, int M, int N, const Acc *__restrict__ W, OutT *C, int64_t ldc,
+    Acc alpha, Acc beta) {
+  const int64_t MN = (int64_t)M * N;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < MN;
+       i += (int64_t)gridDim.x * 256) {
+    Acc s = W[i];
+    for (int z = 1; z < S; ++z) s += W[(int64_t)z * MN + i];
+    const int64_t n = i / M, m = i - n * M;
+    OutT *dst = C + n * ldc + m;
+    Acc v = alpha * s;
+    if (beta != Acc(0)) v = fma_acc(beta, from_out<Acc>(*dst), v);
+    *dst = to_out<OutT>(v);
+  }
+}
+
 }  // namespace hcb

@@ -120,4 +120,34 @@ hcblasStatus_t launch_gemm(hipStream_t stream, int num_cus, GemmParams p,
       stream, p, a_k, b_k, vload, batch);
 }
 
+template <Elem E, typename OutT>
+hcblasStatus_t launch_gemm_splitk(hipStream_t stream, int num_cus,
+                                  GemmParams p, bool a_k, bool b_k, int S,
+                                  int ks, typename ElemT<E>::Acc *W) {
+  using Acc = typename ElemT<E>::Acc;
+  constexpr int BM = 64, BN = 64, BK = TileCfg<E>::BK_SMALL;
+  constexpr int WM = 2, WN = E == Elem::F64 ? 2 : 4;
+  p.tiles_m = (int)cdiv(p.M, BM);
+  p.tiles_n = (int)cdiv(p.N, BN);
+  dim3 grid((unsigned)(p.tiles_m * p.tiles_n), 1, (unsigned)S);
+#define HCB_SPLITK_CASE(AK, BKC)                                           \
+  if (a_k == AK && b_k == BKC)                                             \
+    gemm_splitk_kernel<E, AK, BKC, BM, BN, BK, WM, WN, true, true, 16>     \
+        <<<grid, 64 * WM * WN, 0, stream>>>(p, ks, W);
+  HCB_SPLITK_CASE(false, false)
+  HCB_SPLITK_CASE(false, true)
+  HCB_SPLITK_CASE(true, false)
+  HCB_SPLITK_CASE(true, true)
+#undef HCB_SPLITK_CASE
+  hcblasStatus_t st = launch_status();
+  if (st != HCBLAS_STATUS_SUCCESS) return st;
+  const int64_t MN = (int64_t)p.M * p.N;
+  const int64_t nb = cdiv(MN, 256);
+  const unsigned g = (unsigned)(nb < 4 * num_cus ? nb : 4 * num_cus);
+  splitk_reduce_kernel<Acc, OutT><<<g, 256, 0, stream>>>(
+      S, p.M, p.N, W, static_cast<OutT *>(p.C) + p.cOff, p.ldc,
+      (Acc)p.alpha, (Acc)p.beta);
+  return launch_status();
+}
+
 }  // namespace hcb

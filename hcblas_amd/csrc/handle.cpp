@@ -98,6 +98,7 @@ hcblasStatus_t hcblasDestroy(hcblasHandle_t *handle) {
   if (h->gemm_ws.ptr) (void)hipFree(h->gemm_ws.ptr);
   if (h->split_ws.ptr) (void)hipFree(h->split_ws.ptr);
   if (h->adj_ws.ptr) (void)hipFree(h->adj_ws.ptr);
+  if (h->kpart_ws.ptr) (void)hipFree(h->kpart_ws.ptr);
   delete h;
   *handle = nullptr;
   return HCBLAS_STATUS_SUCCESS;

@@ -370,3 +370,15 @@ def test_reference_large_square(hc, handle, orc, torch_cuda, ta, tb):
     _sampled_check(hc, handle, orc, torch_cuda, "s", ta, tb, n, n, n,
                    seed=n + 11 * ta + 13 * tb)
     torch_cuda.cuda.empty_cache()
+
+
+# Long-K problems with few tiles take the split-K form (partial tiles over K
+# slices, then an in-order reduction with alpha / beta): all trans cases, S
+# and D, an odd shape and one whose K is not a multiple of the slice.
+@pytest.mark.parametrize("p", ["s", "d"])
+@pytest.mark.parametrize("M,N,K", [(256, 128, 16384), (100, 72, 9000)])
+def test_split_k_shapes(hc, handle, orc, torch_cuda, p, M, N, K):
+    for ta in (N_, T_):
+        for tb in (N_, T_):
+            _sampled_check(hc, handle, orc, torch_cuda, p, ta, tb, M, N, K,
+                           seed=M + 5 * N + 3 * K + 17 * ta + 19 * tb)
