@@ -46,8 +46,11 @@ struct hcblasContext {
   // workgroup resets to zero, the device result and a pinned host mirror.
   double *d_partials = nullptr;
   unsigned int *d_ticket = nullptr;
-  double *d_result = nullptr;
+  // Pinned host word of the host-result reductions (dot / asum) and its
+  // device address: the reduction's last workgroup writes the result there
+  // directly, so a call needs no device-to-host copy after the kernel.
   double *h_result = nullptr;
+  double *h_result_dev = nullptr;
 
   // gemv split-reduction partials and per-row-block tickets (grow-only; the
   // ticket region is zeroed when it is (re)allocated and every launch leaves

@@ -63,16 +63,16 @@ hcblasStatus_t hcblasCreate(hcblasHandle_t *handle) {
                 hipSuccess &&
             hipMalloc(&h->d_ticket, sizeof(unsigned int) * kTicketCount) ==
                 hipSuccess &&
-            hipMalloc(&h->d_result, 64) == hipSuccess &&
             hipHostMalloc(&h->h_result, 64, hipHostMallocDefault) ==
                 hipSuccess &&
+            hipHostGetDevicePointer(reinterpret_cast<void **>(&h->h_result_dev),
+                                    h->h_result, 0) == hipSuccess &&
             hipMemset(h->d_ticket, 0, sizeof(unsigned int) * kTicketCount) ==
                 hipSuccess &&
             hipDeviceSynchronize() == hipSuccess;
   if (!ok) {
     if (h->d_partials) (void)hipFree(h->d_partials);
     if (h->d_ticket) (void)hipFree(h->d_ticket);
-    if (h->d_result) (void)hipFree(h->d_result);
     if (h->h_result) (void)hipHostFree(h->h_result);
     delete h;
     return HCBLAS_STATUS_ALLOC_FAILED;
@@ -91,7 +91,6 @@ hcblasStatus_t hcblasDestroy(hcblasHandle_t *handle) {
   h->magic = 0;
   (void)hipFree(h->d_partials);
   (void)hipFree(h->d_ticket);
-  (void)hipFree(h->d_result);
   (void)hipHostFree(h->h_result);
   if (h->gemv_partials.ptr) (void)hipFree(h->gemv_partials.ptr);
   if (h->gemv_tickets.ptr) (void)hipFree(h->gemv_tickets.ptr);

@@ -402,7 +402,7 @@ hcblasStatus_t run_reduce(hcblasHandle_t h, int64_t n, int64_t batch,
                           const T *x, int64_t incx, const T *y, int64_t incy,
                           T *result, Red red) {
   constexpr int VN = V16<T>::n;
-  T *dres = reinterpret_cast<T *>(h->d_result);
+  T *dres = reinterpret_cast<T *>(h->h_result_dev);
   const int64_t flat = n * batch;
   const bool contiguous = incx == 1 && (!Red::two || incy == 1);
   const T *yy = Red::two ? y : x;
@@ -435,10 +435,8 @@ hcblasStatus_t run_reduce(hcblasHandle_t h, int64_t n, int64_t batch,
   hcblasStatus_t st = launch_status();
   if (st != HCBLAS_STATUS_SUCCESS) return st;
   // The API returns a host scalar, so the call completes before returning
-  // (same contract as sdot_array.cpp:136-158).
-  if (hipMemcpyAsync(h->h_result, dres, sizeof(T), hipMemcpyDeviceToHost,
-                     h->stream) != hipSuccess)
-    return HCBLAS_STATUS_EXECUTION_FAILED;
+  // (same contract as sdot_array.cpp:136-158). The kernel wrote the result
+  // into the pinned host word; the kernel's completion makes it visible.
   if (hipStreamSynchronize(h->stream) != hipSuccess)
     return HCBLAS_STATUS_EXECUTION_FAILED;
   *result = *reinterpret_cast<T *>(h->h_result);
