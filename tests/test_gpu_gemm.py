@@ -382,3 +382,52 @@ def test_split_k_shapes(hc, handle, orc, torch_cuda, p, M, N, K):
         for tb in (N_, T_):
             _sampled_check(hc, handle, orc, torch_cuda, p, ta, tb, M, N, K,
                            seed=M + 5 * N + 3 * K + 17 * ta + 19 * tb)
+
+
+@pytest.mark.parametrize("order", [COL, ROW])
+def test_hgemm_split_k(hc, handle, orc, order):
+    """HGEMM with K = 8192 on one tile takes the split-K form (FP32
+    partials, then one rounding to half in the reduction)."""
+    api = hc.api
+    M, N, K = 72, 40, 8200
+    rng = np.random.default_rng(8)
+    A = rng.uniform(-1, 1, M * K).astype(np.float16)
+    B = rng.uniform(-1, 1, K * N).astype(np.float16)
+    C = rng.uniform(-1, 1, M * N).astype(np.float16)
+    for ta in (N_, T_):
+        for tb in (N_, T_):
+            lda = (M if ta == N_ else K) if order == COL else (K if ta == N_ else M)
+            ldb = (K if tb == N_ else N) if order == COL else (N if tb == N_ else K)
+            ldc = M if order == COL else N
+            dC = to_dev(C.view(np.uint16))
+            st = api.hcblas_hgemm(handle, order, ta, tb, M, N, K, 0.75,
+                                  to_dev(A.view(np.uint16)), lda,
+                                  to_dev(B.view(np.uint16)), ldb, -0.5, dC,
+                                  ldc, 0, 0, 0)
+            assert st == 0
+            got = to_host(dC).view(np.uint16).view(np.float16).astype(np.float64)
+            ref = orc.hgemm(order, ta, tb, M, N, K, 0.75, A, lda, B, ldb, -0.5,
+                            C, ldc).astype(np.float64)
+            ulp = np.spacing(np.abs(ref).astype(np.float16)).astype(np.float64)
+            assert np.all(np.abs(got - ref) <= ulp + K * 2 ** -23 * 2), (ta, tb)
+
+
+def test_bf16_split_k_single_problem(hc, handle, torch_cuda):
+    """A single BF16 problem with a long K and few tiles: split-K partials in
+    FP32, reduced in slice order (alpha, beta applied once)."""
+    torch = torch_cuda
+    api = hc.api
+    M, N, K = 192, 128, 16384
+    g = torch.Generator(device="cuda").manual_seed(4)
+    A = (torch.rand(M, K, generator=g, device="cuda") * 2 - 1).bfloat16()
+    B = (torch.rand(K, N, generator=g, device="cuda") * 2 - 1).bfloat16()
+    C0 = torch.rand(M, N, generator=g, device="cuda") * 2 - 1
+    C = C0.clone()
+    st = api.hcblas_gemm_strided_batched(handle, 5, ROW, N_, N_, M, N, K, 1.5,
+                                         A, K, 0, M * K, B, N, 0, K * N, 0.25,
+                                         C, N, 0, M * N, 1)
+    assert st == 0
+    torch.cuda.synchronize()
+    ref = 1.5 * (A.double() @ B.double()) + 0.25 * C0.double()
+    err = (C.double() - ref).abs().max().item()
+    assert err <= K * 2 ** -23 * 4, err
