@@ -431,3 +431,41 @@ def test_bf16_split_k_single_problem(hc, handle, torch_cuda):
     ref = 1.5 * (A.double() @ B.double()) + 0.25 * C0.double()
     err = (C.double() - ref).abs().max().item()
     assert err <= K * 2 ** -23 * 4, err
+
+
+# Shapes large enough for the biggest tiles (256x256 FP32, 256x128 FP64),
+# with beta != 0, so the LDS-staged epilogue of those tiles reads C.
+@pytest.mark.parametrize("p,M,N,K", [("s", 4096, 4096, 256), ("d", 4096, 2048, 64)])
+def test_large_tiles_with_beta(hc, handle, orc, torch_cuda, p, M, N, K):
+    for ta in (N_, T_):
+        for tb in (N_, T_):
+            _sampled_check(hc, handle, orc, torch_cuda, p, ta, tb, M, N, K,
+                           seed=M + 7 * N + K + 23 * ta + 29 * tb)
+
+
+def test_hgemm_large_tiles_with_beta(hc, handle, torch_cuda):
+    """HGEMM on 256x256 tiles (FP16 in and out, FP32 accumulate) with
+    beta != 0: sampled entries against the FP64 value rounded to half."""
+    torch = torch_cuda
+    api = hc.api
+    M, N, K = 4096, 4096, 512
+    g = torch.Generator(device="cuda").manual_seed(12)
+    A = (torch.rand(M * K, generator=g, device="cuda") * 2 - 1).half()
+    B = (torch.rand(K * N, generator=g, device="cuda") * 2 - 1).half()
+    C0 = (torch.rand(M * N, generator=g, device="cuda") * 2 - 1).half()
+    C = C0.clone()
+    st = api.hcblas_hgemm(handle, COL, N_, N_, M, N, K, 0.75, A.data_ptr(), M,
+                          B.data_ptr(), K, 0.5, C.data_ptr(), M, 0, 0, 0)
+    assert st == 0
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(3)
+    rows = torch.as_tensor(rng.integers(0, M, 1024), device="cuda")
+    cols = torch.as_tensor(rng.integers(0, N, 1024), device="cuda")
+    Am = A.view(K, M).t().double()     # A(m, k)
+    Bm = B.view(N, K).t().double()     # B(k, n)
+    ref = 0.75 * (Am[rows] * Bm[:, cols].t()).sum(1) + \
+        0.5 * C0.view(N, M).t().double()[rows, cols]
+    got = C.view(N, M).t()[rows, cols].double().cpu().numpy()
+    ref = ref.cpu().numpy()
+    ulp = np.spacing(np.abs(ref).astype(np.float16)).astype(np.float64)
+    assert np.all(np.abs(got - ref) <= ulp + K * 2 ** -23 * 2)
