@@ -513,33 +513,6 @@ __device__ __forceinline__ void gemm_mainloop(
   }
 }
 
-// Calls f(row, col, value) for every accumulator value of this lane that
-// falls inside M x N (row / col are absolute output coordinates).
-template <Elem E, int BM, int BN, int WM, int WN, int MI, typename AccArr,
-          typename F>
-__device__ __forceinline__ void for_each_acc(const AccArr &acc, int M, int N,
-                                             int m0, int n0, F &&f) {
-  using WT = WaveTile<E, BM, BN, WM, WN, MI>;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int wm = wave / WN, wn = wave % WN;
-  const int lrow = lane % MI, lgrp = lane / MI;
-#pragma unroll
-  for (int i = 0; i < WT::FM; ++i) {
-#pragma unroll
-    for (int j = 0; j < WT::FN; ++j) {
-      const int col = n0 + wn * WT::WTN + j * MI + lrow;
-      if (col >= N) continue;
-      const int rowb = m0 + wm * WT::WTM + i * MI;
-#pragma unroll
-      for (int r = 0; r < WT::NVAL; ++r) {
-        const int row = rowb + out_row<E, MI>(lgrp, r);
-        if (row >= M) continue;
-        f(row, col, acc[i][j][r]);
-      }
-    }
-  }
-}
-
 // Epilogue of a BM x BN tile through LDS. Stored straight from the
 // accumulator layout, one store instruction writes one value per lane into
 // 16 or 32 different columns, so the tile leaves in BM*BN/64 scattered store
