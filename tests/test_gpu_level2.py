@@ -106,6 +106,34 @@ def test_ger_bit_exact(hc, handle, orc, p, order, M, N):
         assert bitwise_equal(to_host(dA), exp)
 
 
+@pytest.mark.parametrize("p,M,N,batch", [("d", 6144, 6144, 1),
+                                         ("s", 8448, 8448, 1),
+                                         ("d", 2048, 2048, 9)])
+def test_ger_streaming_bit_exact(hc, handle, orc, p, M, N, batch):
+    """Matrices above 256 MiB (single and batched) take the non-temporal
+    load / store path; bit-exact against the oracle like the cached one."""
+    api = hc.api
+    gen = orc.uniform_f32 if p == "s" else orc.uniform_f64
+    A = gen(M * N * batch, 7)
+    assert A.nbytes > 256 << 20
+    x, y = gen(M * batch, 8), gen(N * batch, 9)
+    dA = to_dev(A)
+    if batch == 1:
+        st = getattr(api, f"hcblas_{p}ger")(handle, COL, M, N, -0.75,
+                                            to_dev(x), 0, 1, to_dev(y), 0, 1,
+                                            dA, 0, M)
+    else:
+        st = getattr(api, f"hcblas{p.upper()}gerBatched")(
+            handle, M, N, -0.75, to_dev(x), 1, to_dev(y), 1, dA, M, batch)
+    assert st == 0
+    got = to_host(dA)
+    for i in range(batch):
+        sl = slice(i * M * N, (i + 1) * M * N)
+        exp = orc.ger(COL, M, N, -0.75, x[i * M:(i + 1) * M], 1,
+                      y[i * N:(i + 1) * N], 1, A[sl], M)
+        assert bitwise_equal(got[sl], exp)
+
+
 # Shapes that reach the band kernels: (p, order, trans, M, N, batch).
 # Column-contiguous case (col-major T, row-major N): bands of 16 or 8 row
 # steps of 256 x 16 bytes with enough column blocks of 8 to fill the GPU.
