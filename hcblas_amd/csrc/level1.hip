@@ -407,8 +407,12 @@ hcblasStatus_t run_reduce(hcblasHandle_t h, int64_t n, int64_t batch,
   const bool contiguous = incx == 1 && (!Red::two || incy == 1);
   const T *yy = Red::two ? y : x;
   if (contiguous && aligned16(x) && (!Red::two || aligned16(y))) {
-    // Same bytes in flight per lane for one stream (asum) as for two (dot).
-    constexpr int U = Red::two ? kReduceUnroll : 2 * kReduceUnroll;
+    // Same bytes in flight per lane for one stream (asum) as for two (dot);
+    // FP32 asum keeps twice that in flight (SASUM 2^28: 6.05 against 5.86
+    // TB/s), which FP64 asum cannot afford (3.3 TB/s: its FP32-unrolled
+    // accumulators and loads no longer fit the registers).
+    constexpr int U = Red::two ? kReduceUnroll
+                               : (sizeof(T) == 4 ? 4 : 2) * kReduceUnroll;
     const int64_t nvec = flat / VN;
     const int grid = (int)std::max<int64_t>(
         1, std::min<int64_t>(cdiv(nvec, kThreads * U), kReduceGrid));
