@@ -9,10 +9,12 @@ of CBLAS, FP64 and double-double references, the seeded generator) with
 numpy-array helpers, and adds the numpy reference for HGEMM. Each function
 names the reference behaviour it restates in ``refblas.c``.
 
-Parity pinning (see tests/test_oracle.py): every function here is checked
-against the golden fixtures under ``tests/golden`` (bundled-CBLAS outputs on
-the reference tests' own integer recipes, exact-rational brute force) and
-against closed forms.
+Parity pinning (tests/test_oracle.py, tests/test_oracle_pins.py): every
+function here is checked against the golden fixtures under ``tests/golden``
+(bundled-CBLAS outputs on the reference tests' own integer recipes,
+exact-rational brute force), against closed forms, and the accuracy
+references (double-double dot / asum / sampled GEMM, hgemm) against exact
+rational arithmetic, with mutation checks that a plausible mistake fails.
 """
 from __future__ import annotations
 
@@ -302,8 +304,10 @@ def hgemm(order, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc):
     """HGEMM with fp32-or-better accumulation: binary16 inputs, the exact
     product sum in float64, one rounding to binary16. (Reference HGEMM
     accumulates in half, hgemm_NN_kernels.cpp:1372; its tests are compiled
-    out, hcblas_hgemm.cpp:34, so this function's parity is pinned only by
-    the closed forms in tests/test_oracle.py.)"""
+    out, hcblas_hgemm.cpp:34.) Pinned in tests/test_oracle_pins.py against
+    float16(sgemm_ref64(...)) for every order and transpose with padded
+    leading dimensions, exact rationals, and a case where half accumulation
+    differs."""
     a = _op_view(A, ta, M, K, lda, order).astype(np.float64)
     b = _op_view(B, tb, K, N, ldb, order).astype(np.float64)
     r = float(alpha) * (a @ b)

@@ -245,6 +245,7 @@ void orc_ddot_dd(int64_t n, const double *x, int64_t incx, const double *y,
 void orc_dasum_dd(int64_t n, const double *x, int64_t incx, double *hi,
                   double *lo) {
   dd_t s = {0, 0};
+  if (incx <= 0) n = 0; /* netlib dasum.f: non-positive increment -> 0 */
   for (int64_t i = 0; i < n; ++i) s = dd_add_d(s, fabs(x[i * incx]));
   *hi = s.hi;
   *lo = s.lo;
