@@ -90,6 +90,24 @@ inline hcblasStatus_t launch_status() {
                          : HCBLAS_STATUS_EXECUTION_FAILED;
 }
 
+// Operation code of an argument: HCBLAS_OP_N/T/C, with the reference's
+// internal transpose codes NoTrans = 'n' and Trans = 't' (hcblasTranspose,
+// hcblaslib.h:51) as aliases of OP_N and OP_T. -1 for any other value.
+inline int canon_op(int t) {
+  switch (t) {
+    case HCBLAS_OP_N:
+    case HCBLAS_NO_TRANS:
+      return HCBLAS_OP_N;
+    case HCBLAS_OP_T:
+    case HCBLAS_TRANS:
+      return HCBLAS_OP_T;
+    case HCBLAS_OP_C:
+      return HCBLAS_OP_C;
+    default:
+      return -1;
+  }
+}
+
 inline bool aligned16(const void *p) {
   return (reinterpret_cast<uintptr_t>(p) & 15u) == 0;
 }

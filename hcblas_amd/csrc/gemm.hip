@@ -30,9 +30,6 @@ namespace hcb {
 
 namespace {
 
-bool valid_op(hcblasOperation_t t) {
-  return t == HCBLAS_OP_N || t == HCBLAS_OP_T || t == HCBLAS_OP_C;
-}
 
 // Checks dimensions and leading dimensions (CBLAS rules) and builds the
 // column-major canonical problem. Returns SUCCESS or INVALID_VALUE.
@@ -43,7 +40,9 @@ hcblasStatus_t canonicalize(hcblasOrder_t order, hcblasOperation_t ta,
                             int64_t ldb, int64_t bOff, int64_t sB,
                             const void *const *Bp, int64_t ldc, Canon &c) {
   if (M < 0 || N < 0 || K < 0) return HCBLAS_STATUS_INVALID_VALUE;
-  if (!valid_op(ta) || !valid_op(tb)) return HCBLAS_STATUS_INVALID_VALUE;
+  if (canon_op(ta) < 0 || canon_op(tb) < 0) return HCBLAS_STATUS_INVALID_VALUE;
+  ta = static_cast<hcblasOperation_t>(canon_op(ta));
+  tb = static_cast<hcblasOperation_t>(canon_op(tb));
   if (order != HCBLAS_ROW_MAJOR && order != HCBLAS_COL_MAJOR)
     return HCBLAS_STATUS_INVALID_VALUE;
   const bool col = order == HCBLAS_COL_MAJOR;
@@ -320,10 +319,9 @@ hcblasStatus_t gemm_real(hcblasHandle_t h, const Canon &c, double alpha,
   p.alpha = alpha;
   p.beta = beta;
   // 16-byte global loads need every matrix base, offset and leading dimension
-  // (and batch stride) aligned to one vector. Pointer arrays are checked on
-  // the device side by requiring only the offsets/ld to be aligned; their
-  // bases come from cudaMalloc-style allocations (256-byte aligned), which the
-  // reference's callers use (test-blas.cpp:2000-2018).
+  // (and batch stride) aligned to one vector. For pointer arrays the host
+  // checks offsets and leading dimensions only; the kernel checks each
+  // problem's bases and takes the element-wise loaders for a misaligned one.
   constexpr int V = vec_of<T>();
   // An r-contiguous operand also needs its row count to be a whole number
   // of vectors, so that a vector is wholly inside or outside the rows.
@@ -755,15 +753,19 @@ hcblasStatus_t hcblas_dgemm(hcblasHandle_t handle, hcblasOrder_t order,
 }
 hcblasStatus_t hcblas_hgemm(hcblasHandle_t handle, hcblasOrder_t order,
                             hcblasOperation_t typeA, hcblasOperation_t typeB,
-                            int M, int N, int K, const float *alpha,
+                            int M, int N, int K, const hcblasHalf *alpha,
                             const hcblasHalf *A, int64_t lda,
                             const hcblasHalf *B, int64_t ldb,
-                            const float *beta, hcblasHalf *C, int64_t ldc,
+                            const hcblasHalf *beta, hcblasHalf *C, int64_t ldc,
                             int64_t aOffset, int64_t bOffset,
                             int64_t cOffset) {
-  return gemm_common(handle, HCBLAS_R_16F, order, typeA, typeB, M, N, K, alpha,
+  if (handle_ok(handle) && (alpha == nullptr || beta == nullptr))
+    return HCBLAS_STATUS_INVALID_VALUE;
+  const float a = alpha ? half_to_float(*alpha) : 0.f;
+  const float b = beta ? half_to_float(*beta) : 0.f;
+  return gemm_common(handle, HCBLAS_R_16F, order, typeA, typeB, M, N, K, &a,
                      A, lda, aOffset, 0, nullptr, B, ldb, bOffset, 0, nullptr,
-                     beta, C, ldc, cOffset, 0, nullptr, 1);
+                     &b, C, ldc, cOffset, 0, nullptr, 1);
 }
 hcblasStatus_t hcblas_cgemm(hcblasHandle_t handle, hcblasOrder_t order,
                             hcblasOperation_t typeA, hcblasOperation_t typeB,
@@ -882,8 +884,10 @@ hcblasStatus_t hcblas_gemm_strided_batched_backward(
     return HCBLAS_STATUS_INVALID_VALUE;
   if (order != HCBLAS_ROW_MAJOR && order != HCBLAS_COL_MAJOR)
     return HCBLAS_STATUS_INVALID_VALUE;
-  if ((unsigned)typeA > 2u || (unsigned)typeB > 2u)
+  if (canon_op(typeA) < 0 || canon_op(typeB) < 0)
     return HCBLAS_STATUS_INVALID_VALUE;
+  typeA = static_cast<hcblasOperation_t>(canon_op(typeA));
+  typeB = static_cast<hcblasOperation_t>(canon_op(typeB));
   if (M < 0 || N < 0 || K < 0 || batchCount < 0)
     return HCBLAS_STATUS_INVALID_VALUE;
   if (alpha == nullptr || beta == nullptr || dC == nullptr)

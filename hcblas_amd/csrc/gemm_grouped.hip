@@ -286,7 +286,10 @@ hcblasStatus_t build(hcblasHandle_t h, hcblasGroupedPlan *p, int shape,
   return st;
 }
 
-bool valid_op(int t) { return t == HCBLAS_OP_N || t == HCBLAS_OP_T; }
+// N or T; the reference's 'n' / 't' codes are accepted as aliases.
+bool valid_op(int t) {
+  return canon_op(t) == HCBLAS_OP_N || canon_op(t) == HCBLAS_OP_T;
+}
 
 }  // namespace
 
@@ -316,7 +319,8 @@ hcblasStatus_t hcblasGroupedPlanCreate(hcblasHandle_t handle,
       return HCBLAS_STATUS_INVALID_VALUE;
     }
     const bool col = u.order == HCBLAS_COL_MAJOR;
-    const bool an = u.transA == HCBLAS_OP_N, bn = u.transB == HCBLAS_OP_N;
+    const bool an = canon_op(u.transA) == HCBLAS_OP_N;
+    const bool bn = canon_op(u.transB) == HCBLAS_OP_N;
     const int64_t a_rows = col ? (an ? u.M : u.K) : (an ? u.K : u.M);
     const int64_t b_rows = col ? (bn ? u.K : u.N) : (bn ? u.N : u.K);
     const int64_t c_rows = col ? u.M : u.N;

@@ -356,13 +356,15 @@ int full_grid(int64_t work_items) {
 
 template <typename T, typename Op>
 hcblasStatus_t run_elementwise(hcblasHandle_t h, int64_t n, int64_t batch,
-                               const T *x, int64_t incx, T *y, int64_t incy,
-                               Op op) {
+                               const T *x, int64_t incx, int64_t bsx, T *y,
+                               int64_t incy, int64_t bsy, Op op) {
   constexpr int VN = V16<T>::n;
   using V = typename V16<T>::type;
   if (n <= 0 || batch <= 0) return HCBLAS_STATUS_SUCCESS;
-  // A unit-stride batch with stride n is one contiguous vector of n*batch.
-  const bool contiguous = incx == 1 && incy == 1;
+  // A unit-stride batch with batch stride n is one contiguous vector of
+  // n*batch.
+  const bool contiguous =
+      incx == 1 && incy == 1 && (batch == 1 || (bsx == n && bsy == n));
   const int64_t flat = n * batch;
   if (contiguous && aligned16(x) && aligned16(y)) {
     const int64_t nvec = flat / VN;
@@ -392,19 +394,20 @@ hcblasStatus_t run_elementwise(hcblasHandle_t h, int64_t n, int64_t batch,
     return launch_status();
   }
   elementwise_strided_kernel<T, Op>
-      <<<full_grid(flat), kThreads, 0, h->stream>>>(n, batch, x, incx, n, y,
-                                                    incy, n, op);
+      <<<full_grid(flat), kThreads, 0, h->stream>>>(n, batch, x, incx, bsx, y,
+                                                    incy, bsy, op);
   return launch_status();
 }
 
 template <typename T, typename Red>
 hcblasStatus_t run_reduce(hcblasHandle_t h, int64_t n, int64_t batch,
-                          const T *x, int64_t incx, const T *y, int64_t incy,
-                          T *result, Red red) {
+                          const T *x, int64_t incx, int64_t bsx, const T *y,
+                          int64_t incy, int64_t bsy, T *result, Red red) {
   constexpr int VN = V16<T>::n;
   T *dres = reinterpret_cast<T *>(h->h_result_dev);
   const int64_t flat = n * batch;
-  const bool contiguous = incx == 1 && (!Red::two || incy == 1);
+  const bool contiguous = incx == 1 && (!Red::two || incy == 1) &&
+                          (batch == 1 || (bsx == n && (!Red::two || bsy == n)));
   const T *yy = Red::two ? y : x;
   if (contiguous && aligned16(x) && (!Red::two || aligned16(y))) {
     // Same bytes in flight per lane for one stream (asum) as for two (dot);
@@ -433,7 +436,7 @@ hcblasStatus_t run_reduce(hcblasHandle_t h, int64_t n, int64_t batch,
   } else {
     reduce_strided_kernel<T, Red>
         <<<(int)std::min<int64_t>(full_grid(flat), kReduceGrid), kThreads, 0,
-           h->stream>>>(n, batch, x, incx, n, yy, incy, n, h->d_partials,
+           h->stream>>>(n, batch, x, incx, bsx, yy, incy, bsy, h->d_partials,
                         h->d_ticket, dres, red);
   }
   hcblasStatus_t st = launch_status();
@@ -450,44 +453,52 @@ hcblasStatus_t run_reduce(hcblasHandle_t h, int64_t n, int64_t batch,
 // ---------------------------------------------------------------------------
 // Argument checks shared by the C entry points.
 // ---------------------------------------------------------------------------
+// Batch strides: bsx / bsy elements between the vectors of a batch; a
+// negative value selects the C API's stride of n elements (hcblas.cpp:376).
+inline int64_t batch_stride(int64_t bs, int n) { return bs < 0 ? n : bs; }
+
 template <typename T, typename Op>
 hcblasStatus_t l1_update(hcblasHandle_t h, int n, const T *x, int incx, T *y,
-                         int incy, int batch, Op op) {
+                         int incy, int batch, Op op, int64_t bsx = -1,
+                         int64_t bsy = -1) {
   if (!handle_ok(h)) return HCBLAS_STATUS_NOT_INITIALIZED;
   if (batch < 0 || incx == 0 || incy == 0) return HCBLAS_STATUS_INVALID_VALUE;
   if (n <= 0 || batch == 0) return HCBLAS_STATUS_SUCCESS;
   if (x == nullptr || y == nullptr) return HCBLAS_STATUS_INVALID_VALUE;
-  return run_elementwise<T, Op>(h, n, batch, x, incx, y, incy, op);
+  return run_elementwise<T, Op>(h, n, batch, x, incx, batch_stride(bsx, n), y,
+                                incy, batch_stride(bsy, n), op);
 }
 
 template <typename T>
 hcblasStatus_t l1_scal(hcblasHandle_t h, int n, const T *alpha, T *x, int incx,
-                       int batch) {
+                       int batch, int64_t bsx = -1) {
   if (!handle_ok(h)) return HCBLAS_STATUS_NOT_INITIALIZED;
   if (batch < 0) return HCBLAS_STATUS_INVALID_VALUE;
   // BLAS: non-positive increment is a no-op for scal.
   if (n <= 0 || batch == 0 || incx <= 0) return HCBLAS_STATUS_SUCCESS;
   if (alpha == nullptr || x == nullptr) return HCBLAS_STATUS_INVALID_VALUE;
-  return run_elementwise<T, ScalOp<T>>(h, n, batch, x, incx, x, incx,
+  const int64_t bs = batch_stride(bsx, n);
+  return run_elementwise<T, ScalOp<T>>(h, n, batch, x, incx, bs, x, incx, bs,
                                        ScalOp<T>{*alpha});
 }
 
 template <typename T>
 hcblasStatus_t l1_cscal(hcblasHandle_t h, int n, T ar, T ai, bool real_alpha,
-                        T *x, int incx, int batch) {
+                        T *x, int incx, int batch, int64_t bsx = -1) {
   if (!handle_ok(h)) return HCBLAS_STATUS_NOT_INITIALIZED;
   if (batch < 0) return HCBLAS_STATUS_INVALID_VALUE;
   if (n <= 0 || batch == 0 || incx <= 0) return HCBLAS_STATUS_SUCCESS;
   if (x == nullptr) return HCBLAS_STATUS_INVALID_VALUE;
   int64_t total = (int64_t)n * batch;
   cscal_kernel<T><<<full_grid(total), kThreads, 0, h->stream>>>(
-      n, batch, x, incx, n, ar, ai, real_alpha);
+      n, batch, x, incx, batch_stride(bsx, n), ar, ai, real_alpha);
   return launch_status();
 }
 
 template <typename T, typename Red>
 hcblasStatus_t l1_reduce(hcblasHandle_t h, int n, const T *x, int incx,
-                         const T *y, int incy, T *result, int batch, Red red) {
+                         const T *y, int incy, T *result, int batch, Red red,
+                         int64_t bsx = -1, int64_t bsy = -1) {
   if (!handle_ok(h)) return HCBLAS_STATUS_NOT_INITIALIZED;
   if (result == nullptr || batch < 0) return HCBLAS_STATUS_INVALID_VALUE;
   if (Red::two && (incx == 0 || incy == 0)) return HCBLAS_STATUS_INVALID_VALUE;
@@ -498,7 +509,8 @@ hcblasStatus_t l1_reduce(hcblasHandle_t h, int n, const T *x, int incx,
   }
   if (x == nullptr || (Red::two && y == nullptr))
     return HCBLAS_STATUS_INVALID_VALUE;
-  return run_reduce<T, Red>(h, n, batch, x, incx, y, incy, result, red);
+  return run_reduce<T, Red>(h, n, batch, x, incx, batch_stride(bsx, n), y,
+                            incy, batch_stride(bsy, n), result, red);
 }
 
 }  // namespace
@@ -509,14 +521,38 @@ using namespace hcb;
 namespace {
 template <typename T, bool REAL, typename AT, typename CT>
 hcblasStatus_t cscal_entry(hcblasHandle_t handle, int n, const AT *alpha,
-                           CT *x, int incx, int batch) {
+                           CT *x, int incx, int batch, int64_t bsx = -1) {
   if (handle_ok(handle) && alpha == nullptr)
     return HCBLAS_STATUS_INVALID_VALUE;
   const T *a = reinterpret_cast<const T *>(alpha);
   T ar = a ? a[0] : T(0);
   T ai = (a && !REAL) ? a[1] : T(0);
   return l1_cscal<T>(handle, n, ar, ai, REAL, reinterpret_cast<T *>(x), incx,
-                     batch);
+                     batch, bsx);
+}
+
+// Offset forms (Hcblaslibrary overloads): a null base stays null so that the
+// argument checks see it.
+template <typename P>
+P *at(P *p, int64_t off) {
+  return p ? p + off : nullptr;
+}
+
+// axpy: argument checks, then the netlib quick return for alpha == 0 (y is
+// not touched, so Inf / NaN in x do not reach it; saxpy.f, and the
+// reference's saxpy_array.cpp:190).
+template <typename T>
+hcblasStatus_t axpy_entry(hcblasHandle_t handle, int n, const T *alpha,
+                          const T *x, int incx, T *y, int incy, int batch,
+                          int64_t bsx = -1, int64_t bsy = -1) {
+  if (!handle_ok(handle)) return HCBLAS_STATUS_NOT_INITIALIZED;
+  if (alpha == nullptr) return HCBLAS_STATUS_INVALID_VALUE;
+  if (*alpha == T(0)) {
+    if (batch < 0 || incx == 0 || incy == 0) return HCBLAS_STATUS_INVALID_VALUE;
+    return HCBLAS_STATUS_SUCCESS;
+  }
+  return l1_update<T>(handle, n, x, incx, y, incy, batch, AxpyOp<T>{*alpha},
+                      bsx, bsy);
 }
 }  // namespace
 
@@ -548,30 +584,22 @@ hcblasStatus_t hcblasDasumBatched(hcblasHandle_t handle, int n,
 // axpy -------------------------------------------------------------------
 hcblasStatus_t hcblasSaxpy(hcblasHandle_t handle, int n, const float *alpha,
                            const float *x, int incx, float *y, int incy) {
-  if (handle_ok(handle) && alpha == nullptr) return HCBLAS_STATUS_INVALID_VALUE;
-  return l1_update<float>(handle, n, x, incx, y, incy, 1,
-                          AxpyOp<float>{alpha ? *alpha : 0.f});
+  return axpy_entry<float>(handle, n, alpha, x, incx, y, incy, 1);
 }
 hcblasStatus_t hcblasDaxpy(hcblasHandle_t handle, int n, const double *alpha,
                            const double *x, int incx, double *y, int incy) {
-  if (handle_ok(handle) && alpha == nullptr) return HCBLAS_STATUS_INVALID_VALUE;
-  return l1_update<double>(handle, n, x, incx, y, incy, 1,
-                           AxpyOp<double>{alpha ? *alpha : 0.0});
+  return axpy_entry<double>(handle, n, alpha, x, incx, y, incy, 1);
 }
 hcblasStatus_t hcblasSaxpyBatched(hcblasHandle_t handle, int n,
                                   const float *alpha, const float *x, int incx,
                                   float *y, int incy, int batchCount) {
-  if (handle_ok(handle) && alpha == nullptr) return HCBLAS_STATUS_INVALID_VALUE;
-  return l1_update<float>(handle, n, x, incx, y, incy, batchCount,
-                          AxpyOp<float>{alpha ? *alpha : 0.f});
+  return axpy_entry<float>(handle, n, alpha, x, incx, y, incy, batchCount);
 }
 hcblasStatus_t hcblasDaxpyBatched(hcblasHandle_t handle, int n,
                                   const double *alpha, const double *x,
                                   int incx, double *y, int incy,
                                   int batchCount) {
-  if (handle_ok(handle) && alpha == nullptr) return HCBLAS_STATUS_INVALID_VALUE;
-  return l1_update<double>(handle, n, x, incx, y, incy, batchCount,
-                           AxpyOp<double>{alpha ? *alpha : 0.0});
+  return axpy_entry<double>(handle, n, alpha, x, incx, y, incy, batchCount);
 }
 
 // copy -------------------------------------------------------------------
@@ -679,6 +707,200 @@ hcblasStatus_t hcblasZdscalBatched(hcblasHandle_t handle, int n,
                                    const double *alpha, hcDoubleComplex *x,
                                    int incx, int batchCount) {
   return cscal_entry<double, true>(handle, n, alpha, x, incx, batchCount);
+}
+
+// Offset and batch-offset forms (hcblaslib.h:106-135, 319-479) -----------
+hcblasStatus_t hcblas_saxpy(hcblasHandle_t handle, int N, const float *alpha,
+                            const float *X, int incX, float *Y, int incY,
+                            int64_t xOffset, int64_t yOffset) {
+  return axpy_entry<float>(handle, N, alpha, at(X, xOffset), incX,
+                           at(Y, yOffset), incY, 1);
+}
+hcblasStatus_t hcblas_saxpy_batched(hcblasHandle_t handle, int N,
+                                    const float *alpha, const float *X,
+                                    int incX, int64_t X_batchOffset, float *Y,
+                                    int incY, int64_t Y_batchOffset,
+                                    int64_t xOffset, int64_t yOffset,
+                                    int batchSize) {
+  if (X_batchOffset < 0 || Y_batchOffset < 0)
+    return handle_ok(handle) ? HCBLAS_STATUS_INVALID_VALUE
+                             : HCBLAS_STATUS_NOT_INITIALIZED;
+  return axpy_entry<float>(handle, N, alpha, at(X, xOffset), incX,
+                           at(Y, yOffset), incY, batchSize, X_batchOffset,
+                           Y_batchOffset);
+}
+hcblasStatus_t hcblas_daxpy(hcblasHandle_t handle, int N, const double *alpha,
+                            const double *X, int incX, double *Y, int incY,
+                            int64_t xOffset, int64_t yOffset) {
+  return axpy_entry<double>(handle, N, alpha, at(X, xOffset), incX,
+                           at(Y, yOffset), incY, 1);
+}
+hcblasStatus_t hcblas_daxpy_batched(hcblasHandle_t handle, int N,
+                                    const double *alpha, const double *X,
+                                    int incX, int64_t X_batchOffset, double *Y,
+                                    int incY, int64_t Y_batchOffset,
+                                    int64_t xOffset, int64_t yOffset,
+                                    int batchSize) {
+  if (X_batchOffset < 0 || Y_batchOffset < 0)
+    return handle_ok(handle) ? HCBLAS_STATUS_INVALID_VALUE
+                             : HCBLAS_STATUS_NOT_INITIALIZED;
+  return axpy_entry<double>(handle, N, alpha, at(X, xOffset), incX,
+                           at(Y, yOffset), incY, batchSize, X_batchOffset,
+                           Y_batchOffset);
+}
+
+hcblasStatus_t hcblas_sscal(hcblasHandle_t handle, int N, const float *alpha,
+                            float *X, int incX, int64_t xOffset) {
+  return l1_scal<float>(handle, N, alpha, at(X, xOffset), incX, 1);
+}
+hcblasStatus_t hcblas_sscal_batched(hcblasHandle_t handle, int N,
+                                    const float *alpha, float *X, int incX,
+                                    int64_t xOffset, int64_t X_batchOffset,
+                                    int batchSize) {
+  if (X_batchOffset < 0)
+    return handle_ok(handle) ? HCBLAS_STATUS_INVALID_VALUE
+                             : HCBLAS_STATUS_NOT_INITIALIZED;
+  return l1_scal<float>(handle, N, alpha, at(X, xOffset), incX, batchSize,
+                        X_batchOffset);
+}
+hcblasStatus_t hcblas_dscal(hcblasHandle_t handle, int N, const double *alpha,
+                            double *X, int incX, int64_t xOffset) {
+  return l1_scal<double>(handle, N, alpha, at(X, xOffset), incX, 1);
+}
+hcblasStatus_t hcblas_dscal_batched(hcblasHandle_t handle, int N,
+                                    const double *alpha, double *X, int incX,
+                                    int64_t xOffset, int64_t X_batchOffset,
+                                    int batchSize) {
+  if (X_batchOffset < 0)
+    return handle_ok(handle) ? HCBLAS_STATUS_INVALID_VALUE
+                             : HCBLAS_STATUS_NOT_INITIALIZED;
+  return l1_scal<double>(handle, N, alpha, at(X, xOffset), incX, batchSize,
+                         X_batchOffset);
+}
+
+#define HCB_CSCAL_OFFSET(NAME, T, REAL, AT, CT)                               \
+  hcblasStatus_t NAME(hcblasHandle_t handle, int N, const AT *alpha, CT *X,  \
+                      int incX, int64_t xOffset) {                            \
+    return cscal_entry<T, REAL>(handle, N, alpha, at(X, xOffset), incX, 1);   \
+  }                                                                           \
+  hcblasStatus_t NAME##_batched(hcblasHandle_t handle, int N, const AT *alpha, \
+                                CT *X, int incX, int64_t xOffset,             \
+                                int64_t X_batchOffset, int batchSize) {       \
+    if (X_batchOffset < 0)                                                    \
+      return handle_ok(handle) ? HCBLAS_STATUS_INVALID_VALUE                  \
+                               : HCBLAS_STATUS_NOT_INITIALIZED;               \
+    return cscal_entry<T, REAL>(handle, N, alpha, at(X, xOffset), incX,       \
+                                batchSize, X_batchOffset);                    \
+  }
+HCB_CSCAL_OFFSET(hcblas_cscal, float, false, hcComplex, hcComplex)
+HCB_CSCAL_OFFSET(hcblas_zscal, double, false, hcDoubleComplex, hcDoubleComplex)
+HCB_CSCAL_OFFSET(hcblas_csscal, float, true, float, hcComplex)
+HCB_CSCAL_OFFSET(hcblas_zdscal, double, true, double, hcDoubleComplex)
+#undef HCB_CSCAL_OFFSET
+
+hcblasStatus_t hcblas_scopy(hcblasHandle_t handle, int N, const float *X,
+                            int incX, int64_t xOffset, float *Y, int incY,
+                            int64_t yOffset) {
+  return l1_update<float>(handle, N, at(X, xOffset), incX, at(Y, yOffset),
+                          incY, 1, CopyOp<float>{});
+}
+hcblasStatus_t hcblas_scopy_batched(hcblasHandle_t handle, int N,
+                                    const float *X, int incX, int64_t xOffset,
+                                    float *Y, int incY, int64_t yOffset,
+                                    int64_t X_batchOffset,
+                                    int64_t Y_batchOffset, int batchSize) {
+  if (X_batchOffset < 0 || Y_batchOffset < 0)
+    return handle_ok(handle) ? HCBLAS_STATUS_INVALID_VALUE
+                             : HCBLAS_STATUS_NOT_INITIALIZED;
+  return l1_update<float>(handle, N, at(X, xOffset), incX, at(Y, yOffset),
+                          incY, batchSize, CopyOp<float>{}, X_batchOffset,
+                          Y_batchOffset);
+}
+hcblasStatus_t hcblas_dcopy(hcblasHandle_t handle, int N, const double *X,
+                            int incX, int64_t xOffset, double *Y, int incY,
+                            int64_t yOffset) {
+  return l1_update<double>(handle, N, at(X, xOffset), incX, at(Y, yOffset),
+                           incY, 1, CopyOp<double>{});
+}
+hcblasStatus_t hcblas_dcopy_batched(hcblasHandle_t handle, int N,
+                                    const double *X, int incX,
+                                    int64_t xOffset, double *Y, int incY,
+                                    int64_t yOffset, int64_t X_batchOffset,
+                                    int64_t Y_batchOffset, int batchSize) {
+  if (X_batchOffset < 0 || Y_batchOffset < 0)
+    return handle_ok(handle) ? HCBLAS_STATUS_INVALID_VALUE
+                             : HCBLAS_STATUS_NOT_INITIALIZED;
+  return l1_update<double>(handle, N, at(X, xOffset), incX, at(Y, yOffset),
+                           incY, batchSize, CopyOp<double>{}, X_batchOffset,
+                           Y_batchOffset);
+}
+
+hcblasStatus_t hcblas_sdot(hcblasHandle_t handle, int N, const float *X,
+                           int incX, int64_t xOffset, const float *Y, int incY,
+                           int64_t yOffset, float *result) {
+  return l1_reduce<float>(handle, N, at(X, xOffset), incX, at(Y, yOffset),
+                          incY, result, 1, DotRed<float>{});
+}
+hcblasStatus_t hcblas_sdot_batched(hcblasHandle_t handle, int N,
+                                   const float *X, int incX, int64_t xOffset,
+                                   const float *Y, int incY, int64_t yOffset,
+                                   float *result, int64_t X_batchOffset,
+                                   int64_t Y_batchOffset, int batchSize) {
+  if (X_batchOffset < 0 || Y_batchOffset < 0)
+    return handle_ok(handle) ? HCBLAS_STATUS_INVALID_VALUE
+                             : HCBLAS_STATUS_NOT_INITIALIZED;
+  return l1_reduce<float>(handle, N, at(X, xOffset), incX, at(Y, yOffset),
+                          incY, result, batchSize, DotRed<float>{},
+                          X_batchOffset, Y_batchOffset);
+}
+hcblasStatus_t hcblas_ddot(hcblasHandle_t handle, int N, const double *X,
+                           int incX, int64_t xOffset, const double *Y,
+                           int incY, int64_t yOffset, double *result) {
+  return l1_reduce<double>(handle, N, at(X, xOffset), incX, at(Y, yOffset),
+                           incY, result, 1, DotRed<double>{});
+}
+hcblasStatus_t hcblas_ddot_batched(hcblasHandle_t handle, int N,
+                                   const double *X, int incX, int64_t xOffset,
+                                   const double *Y, int incY, int64_t yOffset,
+                                   double *result, int64_t X_batchOffset,
+                                   int64_t Y_batchOffset, int batchSize) {
+  if (X_batchOffset < 0 || Y_batchOffset < 0)
+    return handle_ok(handle) ? HCBLAS_STATUS_INVALID_VALUE
+                             : HCBLAS_STATUS_NOT_INITIALIZED;
+  return l1_reduce<double>(handle, N, at(X, xOffset), incX, at(Y, yOffset),
+                           incY, result, batchSize, DotRed<double>{},
+                           X_batchOffset, Y_batchOffset);
+}
+
+hcblasStatus_t hcblas_sasum(hcblasHandle_t handle, int N, const float *X,
+                            int incX, int64_t xOffset, float *result) {
+  return l1_reduce<float>(handle, N, at(X, xOffset), incX, nullptr, 1, result,
+                          1, AsumRed<float>{});
+}
+hcblasStatus_t hcblas_sasum_batched(hcblasHandle_t handle, int N,
+                                    const float *X, int incX, int64_t xOffset,
+                                    float *result, int64_t X_batchOffset,
+                                    int batchSize) {
+  if (X_batchOffset < 0)
+    return handle_ok(handle) ? HCBLAS_STATUS_INVALID_VALUE
+                             : HCBLAS_STATUS_NOT_INITIALIZED;
+  return l1_reduce<float>(handle, N, at(X, xOffset), incX, nullptr, 1, result,
+                          batchSize, AsumRed<float>{}, X_batchOffset);
+}
+hcblasStatus_t hcblas_dasum(hcblasHandle_t handle, int N, const double *X,
+                            int incX, int64_t xOffset, double *result) {
+  return l1_reduce<double>(handle, N, at(X, xOffset), incX, nullptr, 1, result,
+                           1, AsumRed<double>{});
+}
+hcblasStatus_t hcblas_dasum_batched(hcblasHandle_t handle, int N,
+                                    const double *X, int incX,
+                                    int64_t xOffset, double *result,
+                                    int64_t X_batchOffset, int batchSize) {
+  if (X_batchOffset < 0)
+    return handle_ok(handle) ? HCBLAS_STATUS_INVALID_VALUE
+                             : HCBLAS_STATUS_NOT_INITIALIZED;
+  return l1_reduce<double>(handle, N, at(X, xOffset), incX, nullptr, 1, result,
+                           batchSize, AsumRed<double>{}, X_batchOffset);
 }
 
 }  // extern "C"

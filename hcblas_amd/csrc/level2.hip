@@ -645,9 +645,13 @@ hcblasStatus_t run_gemv(hcblasHandle_t h, bool panel, int64_t R, int64_t Cc,
     return launch_status();
   }
   constexpr int VN = VecT<T>::n;
-  // 16-byte path: aligned base, leading dimension and rows a multiple of VN.
+  // 16-byte path: aligned base, leading dimension and rows a multiple of VN,
+  // and batch strides that keep every problem's A (and x, where the column
+  // kernels read it as vectors) 16-byte aligned.
+  const bool xvec = !panel && incx == 1;
   const bool vec = aligned16(A) && (ld % VN == 0) && (R % VN == 0) &&
-                   (panel || incx != 1 || aligned16(x));
+                   (!xvec || aligned16(x)) &&
+                   (batch == 1 || (sA % VN == 0 && (!xvec || sx % VN == 0)));
   const int64_t target = (int64_t)h->num_cus * 4;  // workgroups to aim for
   const bool nt = (int64_t)sizeof(T) * R * Cc * batch >= kStreamBytes;
   GemvShape s{R, Cc, ld, incx, incy, sA, sx, sy, 1};
@@ -770,15 +774,16 @@ hcblasStatus_t gemv_entry(hcblasHandle_t h, hcblasOrder_t order,
                           const T *alpha, const T *A, int64_t aOffset,
                           int64_t lda, const T *x, int64_t xOffset,
                           int64_t incx, const T *beta, T *y, int64_t yOffset,
-                          int64_t incy, int64_t batch, bool ref_batch_strides) {
+                          int64_t incy, int64_t batch, bool ref_batch_strides,
+                          int64_t sA_in = 0, int64_t sx_in = 0,
+                          int64_t sy_in = 0) {
   if (!handle_ok(h)) return HCBLAS_STATUS_NOT_INITIALIZED;
   if (M < 0 || N < 0 || incx == 0 || incy == 0 || batch < 0)
     return HCBLAS_STATUS_INVALID_VALUE;
   if (order != HCBLAS_ROW_MAJOR && order != HCBLAS_COL_MAJOR)
     return HCBLAS_STATUS_INVALID_VALUE;
-  if (trans != HCBLAS_OP_N && trans != HCBLAS_OP_T && trans != HCBLAS_OP_C)
-    return HCBLAS_STATUS_INVALID_VALUE;
-  const bool notrans = trans == HCBLAS_OP_N;
+  if (canon_op(trans) < 0) return HCBLAS_STATUS_INVALID_VALUE;
+  const bool notrans = canon_op(trans) == HCBLAS_OP_N;
   const int64_t len_x = notrans ? N : M, len_y = notrans ? M : N;
   // Storage is column-major R x Cc.
   const bool col = order == HCBLAS_COL_MAJOR;
@@ -791,11 +796,11 @@ hcblasStatus_t gemv_entry(hcblasHandle_t h, hcblasOrder_t order,
     return HCBLAS_STATUS_INVALID_VALUE;
   if (*alpha != T(0) && len_x > 0 && (A == nullptr || x == nullptr))
     return HCBLAS_STATUS_INVALID_VALUE;
-  // Batch strides as in hcblas.cpp:1008-1019: x by len(x), y by len(y), A by
-  // M*N.
-  int64_t sA = ref_batch_strides ? (int64_t)M * N : 0;
-  int64_t sx = ref_batch_strides ? len_x : 0;
-  int64_t sy = ref_batch_strides ? len_y : 0;
+  // Batch strides as in hcblas.cpp:1008-1019 (x by len(x), y by len(y), A by
+  // M*N), or the caller's explicit batch offsets.
+  int64_t sA = ref_batch_strides ? (int64_t)M * N : sA_in;
+  int64_t sx = ref_batch_strides ? len_x : sx_in;
+  int64_t sy = ref_batch_strides ? len_y : sy_in;
   const bool panel = col == notrans;
   return run_gemv<T>(h, panel, R, Cc, *alpha, A ? A + aOffset : nullptr, lda,
                      x ? x + xOffset : nullptr, incx, *beta, y + yOffset, incy,
@@ -807,7 +812,9 @@ hcblasStatus_t ger_entry(hcblasHandle_t h, hcblasOrder_t order, int M, int N,
                          const T *alpha, const T *x, int64_t xOffset,
                          int64_t incx, const T *y, int64_t yOffset,
                          int64_t incy, T *A, int64_t aOffset, int64_t lda,
-                         int64_t batch) {
+                         int64_t batch, bool ref_batch_strides = true,
+                         int64_t sA_in = 0, int64_t sx_in = 0,
+                         int64_t sy_in = 0) {
   if (!handle_ok(h)) return HCBLAS_STATUS_NOT_INITIALIZED;
   if (M < 0 || N < 0 || incx == 0 || incy == 0 || batch < 0)
     return HCBLAS_STATUS_INVALID_VALUE;
@@ -820,9 +827,12 @@ hcblasStatus_t ger_entry(hcblasHandle_t h, hcblasOrder_t order, int M, int N,
   if (alpha == nullptr || x == nullptr || y == nullptr || A == nullptr)
     return HCBLAS_STATUS_INVALID_VALUE;
   if (*alpha == T(0)) return HCBLAS_STATUS_SUCCESS;  // netlib quick return
-  // Batch strides as in hcblas.cpp:1160-1167: x by M, y by N, A by M*N.
-  const int64_t sA = batch > 1 ? (int64_t)M * N : 0;
-  const int64_t sxv = batch > 1 ? M : 0, syv = batch > 1 ? N : 0;
+  // Batch strides as in hcblas.cpp:1160-1167 (x by M, y by N, A by M*N), or
+  // the caller's explicit batch offsets.
+  const bool multi = batch > 1;
+  const int64_t sA = !multi ? 0 : ref_batch_strides ? (int64_t)M * N : sA_in;
+  const int64_t sxv = !multi ? 0 : ref_batch_strides ? M : sx_in;
+  const int64_t syv = !multi ? 0 : ref_batch_strides ? N : sy_in;
   const T *v = col ? x + xOffset : y + yOffset;
   const T *w = col ? y + yOffset : x + xOffset;
   const int64_t incv = col ? incx : incy, incw = col ? incy : incx;
@@ -985,6 +995,48 @@ hcblasStatus_t hcblas_dger(hcblasHandle_t handle, hcblasOrder_t order, int M,
                            int64_t aOffset, int64_t lda) {
   return ger_entry<double>(handle, order, M, N, alpha, X, xOffset, incX, Y,
                            yOffset, incY, A, aOffset, lda, 1);
+}
+
+hcblasStatus_t hcblas_sgemv_batched(
+    hcblasHandle_t handle, hcblasOrder_t order, hcblasOperation_t type, int M,
+    int N, const float *alpha, const float *A, int64_t aOffset,
+    int64_t A_batchOffset, int64_t lda, const float *X, int64_t xOffset,
+    int64_t X_batchOffset, int64_t incX, const float *beta, float *Y,
+    int64_t yOffset, int64_t Y_batchOffset, int64_t incY, int batchSize) {
+  return gemv_entry<float>(handle, order, type, M, N, alpha, A, aOffset, lda,
+                           X, xOffset, incX, beta, Y, yOffset, incY, batchSize,
+                           false, A_batchOffset, X_batchOffset, Y_batchOffset);
+}
+hcblasStatus_t hcblas_dgemv_batched(
+    hcblasHandle_t handle, hcblasOrder_t order, hcblasOperation_t type, int M,
+    int N, const double *alpha, const double *A, int64_t aOffset,
+    int64_t A_batchOffset, int64_t lda, const double *X, int64_t xOffset,
+    int64_t X_batchOffset, int64_t incX, const double *beta, double *Y,
+    int64_t yOffset, int64_t Y_batchOffset, int64_t incY, int batchSize) {
+  return gemv_entry<double>(handle, order, type, M, N, alpha, A, aOffset, lda,
+                            X, xOffset, incX, beta, Y, yOffset, incY,
+                            batchSize, false, A_batchOffset, X_batchOffset,
+                            Y_batchOffset);
+}
+hcblasStatus_t hcblas_sger_batched(
+    hcblasHandle_t handle, hcblasOrder_t order, int M, int N,
+    const float *alpha, const float *X, int64_t xOffset,
+    int64_t X_batchOffset, int64_t incX, const float *Y, int64_t yOffset,
+    int64_t Y_batchOffset, int64_t incY, float *A, int64_t aOffset,
+    int64_t A_batchOffset, int64_t lda, int batchSize) {
+  return ger_entry<float>(handle, order, M, N, alpha, X, xOffset, incX, Y,
+                          yOffset, incY, A, aOffset, lda, batchSize, false,
+                          A_batchOffset, X_batchOffset, Y_batchOffset);
+}
+hcblasStatus_t hcblas_dger_batched(
+    hcblasHandle_t handle, hcblasOrder_t order, int M, int N,
+    const double *alpha, const double *X, int64_t xOffset,
+    int64_t X_batchOffset, int64_t incX, const double *Y, int64_t yOffset,
+    int64_t Y_batchOffset, int64_t incY, double *A, int64_t aOffset,
+    int64_t A_batchOffset, int64_t lda, int batchSize) {
+  return ger_entry<double>(handle, order, M, N, alpha, X, xOffset, incX, Y,
+                           yOffset, incY, A, aOffset, lda, batchSize, false,
+                           A_batchOffset, X_batchOffset, Y_batchOffset);
 }
 
 hcblasStatus_t hcblas_sger_backward(hcblasHandle_t handle,

@@ -73,6 +73,16 @@ typedef enum hcblasOperation_t {
   HCBLAS_OP_C = 2
 } hcblasOperation_t;
 
+/* The reference's internal transpose codes (hcblasTranspose, hcblaslib.h:51:
+ * NoTrans = 'n', Trans = 't'). Every entry point that takes an
+ * hcblasOperation_t also accepts these, as aliases of HCBLAS_OP_N and
+ * HCBLAS_OP_T, so callers written against the Hcblaslibrary overloads pass
+ * their codes unchanged. */
+typedef enum hcblasTranspose_t {
+  HCBLAS_NO_TRANS = 'n',
+  HCBLAS_TRANS = 't'
+} hcblasTranspose_t;
+
 /* Storage order, same values as hcblasOrder in hcblaslib.h:47
  * (RowMajor = 0, ColMajor = 1). */
 typedef enum hcblasOrder_t {
@@ -91,6 +101,16 @@ typedef struct hcDoubleComplex {
 
 /* IEEE-754 binary16 bit pattern (reference: hc::half, hcblas.h:687). */
 typedef uint16_t hcblasHalf;
+
+/* Element type of the entries that take untyped (void *) operands. */
+typedef enum hcblasDatatype_t {
+  HCBLAS_R_32F = 0,
+  HCBLAS_R_64F = 1,
+  HCBLAS_C_32F = 2,
+  HCBLAS_C_64F = 3,
+  HCBLAS_R_16F = 4,
+  HCBLAS_R_16BF = 5
+} hcblasDatatype_t;
 
 /* GEMM arithmetic selected by the handle (no reference counterpart).
  *   DEFAULT          : plain FP32 / FP64 / FP16->FP32 MFMA kernels.
@@ -257,6 +277,120 @@ hcblasStatus_t hcblasZdscalBatched(hcblasHandle_t handle, int n,
                                    const double *alpha, hcDoubleComplex *x,
                                    int incx, int batchCount);
 
+/* Level 1 with per-vector offsets and explicit batch offsets: C forms of
+ * the Hcblaslibrary overloads, same argument order (the accelerator_view
+ * argument becomes the handle, `const T &alpha` a host pointer, the dot /
+ * asum output reference a host pointer). Vector i of a batch starts at
+ * X + xOffset + i*X_batchOffset. Batched dot / asum return ONE scalar summed
+ * over the batch, as the reference does (sdot_array.cpp:267-277). Output
+ * batches that overlap race; input batches may overlap.
+ *   axpy  hcblaslib.h:106-135     scal  hcblaslib.h:319-404
+ *   copy  hcblaslib.h:406-436     dot   hcblaslib.h:438-464
+ *   asum  hcblaslib.h:466-479 */
+hcblasStatus_t hcblas_saxpy(hcblasHandle_t handle, int N, const float *alpha,
+                            const float *X, int incX, float *Y, int incY,
+                            int64_t xOffset, int64_t yOffset);
+hcblasStatus_t hcblas_saxpy_batched(hcblasHandle_t handle, int N,
+                                    const float *alpha, const float *X,
+                                    int incX, int64_t X_batchOffset, float *Y,
+                                    int incY, int64_t Y_batchOffset,
+                                    int64_t xOffset, int64_t yOffset,
+                                    int batchSize);
+hcblasStatus_t hcblas_daxpy(hcblasHandle_t handle, int N, const double *alpha,
+                            const double *X, int incX, double *Y, int incY,
+                            int64_t xOffset, int64_t yOffset);
+hcblasStatus_t hcblas_daxpy_batched(hcblasHandle_t handle, int N,
+                                    const double *alpha, const double *X,
+                                    int incX, int64_t X_batchOffset, double *Y,
+                                    int incY, int64_t Y_batchOffset,
+                                    int64_t xOffset, int64_t yOffset,
+                                    int batchSize);
+hcblasStatus_t hcblas_sscal(hcblasHandle_t handle, int N, const float *alpha,
+                            float *X, int incX, int64_t xOffset);
+hcblasStatus_t hcblas_sscal_batched(hcblasHandle_t handle, int N,
+                                    const float *alpha, float *X, int incX,
+                                    int64_t xOffset, int64_t X_batchOffset,
+                                    int batchSize);
+hcblasStatus_t hcblas_dscal(hcblasHandle_t handle, int N, const double *alpha,
+                            double *X, int incX, int64_t xOffset);
+hcblasStatus_t hcblas_dscal_batched(hcblasHandle_t handle, int N,
+                                    const double *alpha, double *X, int incX,
+                                    int64_t xOffset, int64_t X_batchOffset,
+                                    int batchSize);
+hcblasStatus_t hcblas_cscal(hcblasHandle_t handle, int N,
+                            const hcComplex *alpha, hcComplex *X, int incX,
+                            int64_t xOffset);
+hcblasStatus_t hcblas_cscal_batched(hcblasHandle_t handle, int N,
+                                    const hcComplex *alpha, hcComplex *X,
+                                    int incX, int64_t xOffset,
+                                    int64_t X_batchOffset, int batchSize);
+hcblasStatus_t hcblas_zscal(hcblasHandle_t handle, int N,
+                            const hcDoubleComplex *alpha, hcDoubleComplex *X,
+                            int incX, int64_t xOffset);
+hcblasStatus_t hcblas_zscal_batched(hcblasHandle_t handle, int N,
+                                    const hcDoubleComplex *alpha,
+                                    hcDoubleComplex *X, int incX,
+                                    int64_t xOffset, int64_t X_batchOffset,
+                                    int batchSize);
+hcblasStatus_t hcblas_csscal(hcblasHandle_t handle, int N, const float *alpha,
+                             hcComplex *X, int incX, int64_t xOffset);
+hcblasStatus_t hcblas_csscal_batched(hcblasHandle_t handle, int N,
+                                     const float *alpha, hcComplex *X,
+                                     int incX, int64_t xOffset,
+                                     int64_t X_batchOffset, int batchSize);
+hcblasStatus_t hcblas_zdscal(hcblasHandle_t handle, int N, const double *alpha,
+                             hcDoubleComplex *X, int incX, int64_t xOffset);
+hcblasStatus_t hcblas_zdscal_batched(hcblasHandle_t handle, int N,
+                                     const double *alpha, hcDoubleComplex *X,
+                                     int incX, int64_t xOffset,
+                                     int64_t X_batchOffset, int batchSize);
+hcblasStatus_t hcblas_scopy(hcblasHandle_t handle, int N, const float *X,
+                            int incX, int64_t xOffset, float *Y, int incY,
+                            int64_t yOffset);
+hcblasStatus_t hcblas_scopy_batched(hcblasHandle_t handle, int N,
+                                    const float *X, int incX, int64_t xOffset,
+                                    float *Y, int incY, int64_t yOffset,
+                                    int64_t X_batchOffset,
+                                    int64_t Y_batchOffset, int batchSize);
+hcblasStatus_t hcblas_dcopy(hcblasHandle_t handle, int N, const double *X,
+                            int incX, int64_t xOffset, double *Y, int incY,
+                            int64_t yOffset);
+hcblasStatus_t hcblas_dcopy_batched(hcblasHandle_t handle, int N,
+                                    const double *X, int incX,
+                                    int64_t xOffset, double *Y, int incY,
+                                    int64_t yOffset, int64_t X_batchOffset,
+                                    int64_t Y_batchOffset, int batchSize);
+/* `result` is the reference's `dot` output argument. */
+hcblasStatus_t hcblas_sdot(hcblasHandle_t handle, int N, const float *X,
+                           int incX, int64_t xOffset, const float *Y, int incY,
+                           int64_t yOffset, float *result);
+hcblasStatus_t hcblas_sdot_batched(hcblasHandle_t handle, int N,
+                                   const float *X, int incX, int64_t xOffset,
+                                   const float *Y, int incY, int64_t yOffset,
+                                   float *result, int64_t X_batchOffset,
+                                   int64_t Y_batchOffset, int batchSize);
+hcblasStatus_t hcblas_ddot(hcblasHandle_t handle, int N, const double *X,
+                           int incX, int64_t xOffset, const double *Y,
+                           int incY, int64_t yOffset, double *result);
+hcblasStatus_t hcblas_ddot_batched(hcblasHandle_t handle, int N,
+                                   const double *X, int incX, int64_t xOffset,
+                                   const double *Y, int incY, int64_t yOffset,
+                                   double *result, int64_t X_batchOffset,
+                                   int64_t Y_batchOffset, int batchSize);
+/* `result` is the reference's `Y` output argument (a host scalar). */
+hcblasStatus_t hcblas_sasum(hcblasHandle_t handle, int N, const float *X,
+                            int incX, int64_t xOffset, float *result);
+hcblasStatus_t hcblas_sasum_batched(hcblasHandle_t handle, int N,
+                                    const float *X, int incX, int64_t xOffset,
+                                    float *result, int64_t X_batchOffset,
+                                    int batchSize);
+hcblasStatus_t hcblas_dasum(hcblasHandle_t handle, int N, const double *X,
+                            int incX, int64_t xOffset, double *result);
+hcblasStatus_t hcblas_dasum_batched(hcblasHandle_t handle, int N,
+                                    const double *X, int incX,
+                                    int64_t xOffset, double *result,
+                                    int64_t X_batchOffset, int batchSize);
+
 /* ------------------------------------------------------------------------ */
 /* Level 2 (column-major, hcblas.h:516-586)                                 */
 /* Batched strides: x = len(x), y = len(y), A = m*n (hcblas.cpp:1017-1019,  */
@@ -325,6 +459,37 @@ hcblasStatus_t hcblas_dger(hcblasHandle_t handle, hcblasOrder_t order, int M,
                            int64_t xOffset, int64_t incX, const double *Y,
                            int64_t yOffset, int64_t incY, double *A,
                            int64_t aOffset, int64_t lda);
+
+/* Explicit order, offsets and batch offsets: C form of the batch overloads
+ * Hcblaslibrary::hcblas_{s,d}gemv (hcblaslib.h:187-190, 206-212) and
+ * hcblas_{s,d}ger (hcblaslib.h:148-156, 167-175), same argument order.
+ * Problem i uses A + aOffset + i*A_batchOffset, X + xOffset +
+ * i*X_batchOffset and Y + yOffset + i*Y_batchOffset. Output batches that
+ * overlap race. */
+hcblasStatus_t hcblas_sgemv_batched(
+    hcblasHandle_t handle, hcblasOrder_t order, hcblasOperation_t type, int M,
+    int N, const float *alpha, const float *A, int64_t aOffset,
+    int64_t A_batchOffset, int64_t lda, const float *X, int64_t xOffset,
+    int64_t X_batchOffset, int64_t incX, const float *beta, float *Y,
+    int64_t yOffset, int64_t Y_batchOffset, int64_t incY, int batchSize);
+hcblasStatus_t hcblas_dgemv_batched(
+    hcblasHandle_t handle, hcblasOrder_t order, hcblasOperation_t type, int M,
+    int N, const double *alpha, const double *A, int64_t aOffset,
+    int64_t A_batchOffset, int64_t lda, const double *X, int64_t xOffset,
+    int64_t X_batchOffset, int64_t incX, const double *beta, double *Y,
+    int64_t yOffset, int64_t Y_batchOffset, int64_t incY, int batchSize);
+hcblasStatus_t hcblas_sger_batched(
+    hcblasHandle_t handle, hcblasOrder_t order, int M, int N,
+    const float *alpha, const float *X, int64_t xOffset,
+    int64_t X_batchOffset, int64_t incX, const float *Y, int64_t yOffset,
+    int64_t Y_batchOffset, int64_t incY, float *A, int64_t aOffset,
+    int64_t A_batchOffset, int64_t lda, int batchSize);
+hcblasStatus_t hcblas_dger_batched(
+    hcblasHandle_t handle, hcblasOrder_t order, int M, int N,
+    const double *alpha, const double *X, int64_t xOffset,
+    int64_t X_batchOffset, int64_t incX, const double *Y, int64_t yOffset,
+    int64_t Y_batchOffset, int64_t incY, double *A, int64_t aOffset,
+    int64_t A_batchOffset, int64_t lda, int batchSize);
 
 /* ------------------------------------------------------------------------ */
 /* Level 3 (column-major C API, hcblas.h:662-790)                           */
@@ -409,7 +574,11 @@ hcblasStatus_t hcblasHgemmBatched(hcblasHandle_t handle,
 
 /* Explicit order and per-matrix offsets: C form of
  * Hcblaslibrary::hcblas_{s,d,h,c,z}gemm (hcblaslib.h:215-242, 268-293).
- * Argument order follows the reference overloads. */
+ * Argument order follows the reference overloads; typeA / typeB take
+ * HCBLAS_OP_N/T/C or the reference's 'n' / 't'. The return value is the
+ * hcblasStatus_t of the C API: HCBLAS_STATUS_SUCCESS where the overload
+ * returned HCBLAS_SUCCEEDS, a non-zero code (INVALID_VALUE for a bad
+ * argument) where it returned HCBLAS_INVALID. */
 hcblasStatus_t hcblas_sgemm(hcblasHandle_t handle, hcblasOrder_t order,
                             hcblasOperation_t typeA, hcblasOperation_t typeB,
                             int M, int N, int K, const float *alpha,
@@ -424,12 +593,15 @@ hcblasStatus_t hcblas_dgemm(hcblasHandle_t handle, hcblasOrder_t order,
                             int64_t ldb, const double *beta, double *C,
                             int64_t ldc, int64_t aOffset, int64_t bOffset,
                             int64_t cOffset);
+/* alpha / beta are binary16, as the reference's `const hc::half &`
+ * (hcblaslib.h:236-238); they are widened exactly to FP32 for the
+ * FP32-accumulating kernel. */
 hcblasStatus_t hcblas_hgemm(hcblasHandle_t handle, hcblasOrder_t order,
                             hcblasOperation_t typeA, hcblasOperation_t typeB,
-                            int M, int N, int K, const float *alpha,
+                            int M, int N, int K, const hcblasHalf *alpha,
                             const hcblasHalf *A, int64_t lda,
                             const hcblasHalf *B, int64_t ldb,
-                            const float *beta, hcblasHalf *C, int64_t ldc,
+                            const hcblasHalf *beta, hcblasHalf *C, int64_t ldc,
                             int64_t aOffset, int64_t bOffset, int64_t cOffset);
 hcblasStatus_t hcblas_cgemm(hcblasHandle_t handle, hcblasOrder_t order,
                             hcblasOperation_t typeA, hcblasOperation_t typeB,
@@ -452,15 +624,6 @@ hcblasStatus_t hcblas_zgemm(hcblasHandle_t handle, hcblasOrder_t order,
  * the reference never reaches (sgemm_array_kernels_rMajor_batch.cpp,
  * sgemm_array_wrapper.cpp:113-151). dtype selects the element type of A, B, C
  * (alpha/beta point to that type; for HCBLAS_R_16F they point to float). */
-typedef enum hcblasDatatype_t {
-  HCBLAS_R_32F = 0,
-  HCBLAS_R_64F = 1,
-  HCBLAS_C_32F = 2,
-  HCBLAS_C_64F = 3,
-  HCBLAS_R_16F = 4,
-  HCBLAS_R_16BF = 5
-} hcblasDatatype_t;
-
 hcblasStatus_t hcblas_gemm_strided_batched(
     hcblasHandle_t handle, hcblasDatatype_t dtype, hcblasOrder_t order,
     hcblasOperation_t typeA, hcblasOperation_t typeB, int M, int N, int K,

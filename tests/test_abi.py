@@ -10,6 +10,7 @@ lib/include/hcblas.h, lib/src/hcblas.cpp; status codes hcblas.h:56-65).
 - the status / operation / order enum values match the reference.
 """
 import ctypes
+import os
 import re
 import subprocess
 
@@ -163,3 +164,35 @@ def test_hipblas_shim_exports_and_null_handle():
     assert lib.hipblasSgemm(None, 0, hipblas_shim.OP_N, 1, 1, 1, None, None, 1,
                             None, 1, None, None, 1) == 3
     assert lib.hipblasCreate(None) == 3
+
+
+def test_hcblaslib_mirror_compiles_host_only(tmp_path):
+    """include/hcblaslib.hpp is host code over the C ABI: it compiles with
+    g++ alone (no HIP headers), and its overloads resolve for the
+    reference's call forms (a dot with a float& result, a pointer-array
+    GEMM, the 'n' / 't' transpose codes)."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    src = tmp_path / "use.cpp"
+    src.write_text(
+        '#include "hcblaslib.hpp"\n'
+        "static_assert(NoTrans == 'n' && Trans == 't', \"codes\");\n"
+        "static_assert(HCBLAS_SUCCEEDS == 0 && HCBLAS_INVALID == -1, \"st\");\n"
+        "static_assert((int)HCBLAS_NO_TRANS == 'n' && (int)HCBLAS_TRANS == 't',"
+        " \"alias\");\n"
+        "int use(Hcblaslibrary &hc, float *x, float *y, float **pa) {\n"
+        "  float r = 0; hipStream_t s = hc.currentAcclView;\n"
+        "  int st = hc.hcblas_sdot(s, 4, x, 1, 0, y, 1, 0, r);\n"
+        "  st += hc.hcblas_sgemm(s, ColMajor, NoTrans, Trans, 2, 2, 2, 1.f, pa,"
+        " 2, 0, pa, 2, 0, 0.f, pa, 2, 0, 0, 0, 0, 1);\n"
+        "  st += hc.hcblas_saxpy(s, 4, 2.f, x, 1, 4, y, 1, 4, 0, 0, 3);\n"
+        "  return st + (int)r;\n"
+        "}\n")
+    inc = os.path.join(os.path.dirname(os.path.dirname(
+        os.path.abspath(__file__))), "include")
+    p = subprocess.run([gxx, "-std=c++17", "-Wall", "-Werror", "-fsyntax-only",
+                        "-I", inc, str(src)], capture_output=True, text=True)
+    assert p.returncode == 0, p.stderr
