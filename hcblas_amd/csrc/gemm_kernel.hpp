@@ -642,8 +642,14 @@ __device__ __forceinline__ void gemm_tile(
       acc, C, ldc, M, N, m0, n0, alpha_d, beta_d, lds, vec_c);
 }
 
+// CHK (pointer-array batches with 16-byte loads): the matrices of a batch
+// come from a device array of pointers that the host never sees, so each
+// workgroup checks its problem's A and B bases (one problem per blockIdx.z,
+// so the branch is uniform) and runs the element-wise loaders for a base
+// that is not 16-byte aligned. Offsets and leading dimensions were checked
+// on the host.
 template <Elem E, typename OutT, bool A_K, bool B_K, int BM, int BN, int BK,
-          int WM, int WN, bool VA, bool VB, int MI = 16>
+          int WM, int WN, bool VA, bool VB, int MI = 16, bool CHK = false>
 __global__ __launch_bounds__(64 * WM * WN) void gemm_mfma_kernel(
     GemmParams p) {
   using T = typename ElemT<E>::T;
@@ -666,6 +672,15 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_mfma_kernel(
     C = static_cast<OutT *>(p.C) + p.cOff + (int64_t)z * p.strideC;
   int tile_m, tile_n;
   tile_coords(blockIdx.x, p.tiles_m, p.tiles_n, tile_m, tile_n);
+  if constexpr (CHK) {
+    if (((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(B)) &
+         15u) != 0) {
+      gemm_tile<E, OutT, A_K, B_K, BM, BN, BK, WM, WN, false, false, MI>(
+          A, p.lda, B, p.ldb, C, p.ldc, p.M, p.N, p.K, p.alpha, p.beta,
+          tile_m * BM, tile_n * BN, lds);
+      return;
+    }
+  }
   gemm_tile<E, OutT, A_K, B_K, BM, BN, BK, WM, WN, VA, VB, MI>(
       A, p.lda, B, p.ldb, C, p.ldc, p.M, p.N, p.K, p.alpha, p.beta,
       tile_m * BM, tile_n * BN, lds);

@@ -45,18 +45,25 @@ hcblasStatus_t launch_tiles(hipStream_t stream, GemmParams p, bool a_k,
       else q.C = (char *)q.C + b0 * q.strideC * sizeof(OutT);
     }
     dim3 grid(tiles, 1, (unsigned)nb);
-#define HCB_GEMM_CASE(AK, BKC, VL)                                        \
-  if (a_k == AK && b_k == BKC && vload == VL)                             \
-    gemm_mfma_kernel<E, OutT, AK, BKC, BM, BN, BK, WM, WN, VL, VL, MI>    \
+    // Pointer-array batches with 16-byte loads check each problem's bases
+    // on the device (CHK).
+    const bool chk = vload && (q.Aptrs || q.Bptrs);
+#define HCB_GEMM_CASE(AK, BKC, VL, CK)                                    \
+  if (a_k == AK && b_k == BKC && vload == VL && chk == CK)                \
+    gemm_mfma_kernel<E, OutT, AK, BKC, BM, BN, BK, WM, WN, VL, VL, MI, CK> \
         <<<grid, 64 * WM * WN, 0, stream>>>(q);
-    HCB_GEMM_CASE(false, false, false)
-    HCB_GEMM_CASE(false, true, false)
-    HCB_GEMM_CASE(true, false, false)
-    HCB_GEMM_CASE(true, true, false)
-    HCB_GEMM_CASE(false, false, true)
-    HCB_GEMM_CASE(false, true, true)
-    HCB_GEMM_CASE(true, false, true)
-    HCB_GEMM_CASE(true, true, true)
+    HCB_GEMM_CASE(false, false, false, false)
+    HCB_GEMM_CASE(false, true, false, false)
+    HCB_GEMM_CASE(true, false, false, false)
+    HCB_GEMM_CASE(true, true, false, false)
+    HCB_GEMM_CASE(false, false, true, false)
+    HCB_GEMM_CASE(false, true, true, false)
+    HCB_GEMM_CASE(true, false, true, false)
+    HCB_GEMM_CASE(true, true, true, false)
+    HCB_GEMM_CASE(false, false, true, true)
+    HCB_GEMM_CASE(false, true, true, true)
+    HCB_GEMM_CASE(true, false, true, true)
+    HCB_GEMM_CASE(true, true, true, true)
 #undef HCB_GEMM_CASE
     hcblasStatus_t st = launch_status();
     if (st != HCBLAS_STATUS_SUCCESS) return st;
@@ -81,6 +88,10 @@ hcblasStatus_t launch_gemm(hipStream_t stream, int num_cus, GemmParams p,
   auto tiles = [&](int bm, int bn) {
     return cdiv(p.M, bm) * cdiv(p.N, bn) * batch;
   };
+  // Tiles whose element-wise loaders do not fit the registers take aligned
+  // strided operands only: a pointer-array batch may hold misaligned bases,
+  // which its kernel sends to the element-wise loaders.
+  const bool big_ok = vload && !p.Aptrs && !p.Bptrs;
   if constexpr (E == Elem::F32) {
     if (tiles(256, 256) >= num_cus)
       return launch_tiles<E, OutT, 256, 256, 32, 2, 4, 32>(stream, p, a_k, b_k,
@@ -92,7 +103,7 @@ hcblasStatus_t launch_gemm(hipStream_t stream, int num_cus, GemmParams p,
       return launch_tiles<E, OutT, 128, 128, 32, 2, 2>(stream, p, a_k, b_k,
                                                        vload, batch);
   } else if constexpr (E == Elem::F64) {
-    if (vload && tiles(256, 128) >= num_cus)
+    if (big_ok && tiles(256, 128) >= num_cus)
       return launch_tiles<E, OutT, 256, 128, 16, 4, 2>(stream, p, a_k, b_k,
                                                        vload, batch);
     if (tiles(128, 64) >= num_cus)
@@ -101,7 +112,7 @@ hcblasStatus_t launch_gemm(hipStream_t stream, int num_cus, GemmParams p,
   } else {
     // (the element-wise loaders of unaligned operands do not fit the 256x256
     // tile's registers: they take the 128x128 tile)
-    if (vload && tiles(256, 256) >= num_cus)
+    if (big_ok && tiles(256, 256) >= num_cus)
       return launch_tiles<E, OutT, 256, 256, TileCfg<E>::BK_BIG, 2, 4, 32>(
           stream, p, a_k, b_k, vload, batch);
     if (tiles(128, 128) >= num_cus)

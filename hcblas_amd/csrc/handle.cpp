@@ -151,8 +151,9 @@ hcblasStatus_t hcblasSetOzakiModuli(hcblasHandle_t handle, int moduli) {
 // Set/Get Vector/Matrix. The reference copies n*elemSize bytes contiguously
 // and reports MAPPING_ERROR when that overruns the device allocation
 // (hcblas.cpp:160-168,252-260). Here strides and leading dimensions are
-// honoured through hipMemcpy2D, and the overrun check uses the allocation
-// range reported by hipMemGetAddressRange.
+// honoured through one pitched copy (hipMemcpy2DAsync) on the handle's
+// stream, and the overrun check uses the allocation range reported by
+// hipMemGetAddressRange.
 // ---------------------------------------------------------------------------
 namespace {
 
@@ -184,31 +185,16 @@ hcblasStatus_t copy_strided(hcblasHandle_t h, int rows, int cols, int elem,
   if (dev_pitch == rows && host_pitch == rows) {
     // Dense on both sides: one copy.
     e = hipMemcpyAsync(dst, src, (size_t)rows * cols * elem, kind, h->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   } else {
-    // Strided: stage the device span on the host, scatter / gather there.
-    std::vector<char> stage(span);
-    const size_t rb = (size_t)rows * elem;
-    if (to_device) {
-      e = hipMemcpy(stage.data(), dst, span, hipMemcpyDeviceToHost);
-      if (e == hipSuccess)
-        for (int c = 0; c < cols; ++c)
-          std::memcpy(stage.data() + (size_t)c * dev_pitch * elem,
-                      static_cast<const char *>(src) +
-                          (size_t)c * host_pitch * elem,
-                      rb);
-      if (e == hipSuccess)
-        e = hipMemcpy(dst, stage.data(), span, hipMemcpyHostToDevice);
-    } else {
-      e = hipStreamSynchronize(h->stream);
-      if (e == hipSuccess)
-        e = hipMemcpy(stage.data(), src, span, hipMemcpyDeviceToHost);
-      if (e == hipSuccess)
-        for (int c = 0; c < cols; ++c)
-          std::memcpy(static_cast<char *>(dst) + (size_t)c * host_pitch * elem,
-                      stage.data() + (size_t)c * dev_pitch * elem, rb);
-    }
+    // Strided: one pitched copy of cols runs of rows elements, ordered on
+    // the handle's stream after the work already enqueued there (so a
+    // kernel still writing the gaps between the runs is not raced).
+    const size_t spitch = (size_t)spitch_elems * elem;
+    const size_t dpitch = (size_t)dpitch_elems * elem;
+    e = hipMemcpy2DAsync(dst, dpitch, src, spitch, (size_t)rows * elem,
+                         (size_t)cols, kind, h->stream);
   }
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   return e == hipSuccess ? HCBLAS_STATUS_SUCCESS
                          : HCBLAS_STATUS_MAPPING_ERROR;
 }

@@ -11,13 +11,14 @@
 //       A_out = alpha * x * y^T + A (row-major, float32 / float64)
 //       backward: hcblas_{s,d}ger_backward (dx, dy), dA = dA_out.
 //
-// Work is enqueued on torch's current HIP stream through a per-thread,
-// per-device library handle. Built as a separate shared object linked
+// Work is enqueued on torch's current HIP stream through a library handle
+// kept per thread, device and stream. Built as a separate shared object linked
 // against libhcblas.so and libtorch; loaded with torch.ops.load_library.
 #include <torch/torch.h>
 #include <torch/library.h>
 #include <c10/hip/HIPStream.h>
 
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
@@ -35,19 +36,36 @@ void check(hcblasStatus_t st, const char *what) {
               static_cast<int>(st));
 }
 
-// One handle per (thread, device), bound to torch's current stream.
+// One handle per (thread, device, stream): a handle's workspaces (split-K
+// partials, the FP32 operand copies of a BF16 backward) are used by the
+// work it enqueues, so two streams never share one handle, and work on one
+// stream cannot overwrite a workspace that another stream is still reading.
+struct StreamKey {
+  int device;
+  hipStream_t stream;
+  bool operator==(const StreamKey &o) const {
+    return device == o.device && stream == o.stream;
+  }
+};
+struct StreamKeyHash {
+  size_t operator()(const StreamKey &k) const {
+    return std::hash<const void *>()(k.stream) * 31u + (size_t)k.device;
+  }
+};
+
 hcblasHandle_t handle_for(const Tensor &t) {
-  thread_local std::unordered_map<int, hcblasHandle_t> handles;
+  thread_local std::unordered_map<StreamKey, hcblasHandle_t, StreamKeyHash>
+      handles;
   const int dev = t.get_device();
-  auto it = handles.find(dev);
+  const StreamKey key{dev, c10::hip::getCurrentHIPStream(dev).stream()};
+  auto it = handles.find(key);
   if (it == handles.end()) {
     c10::DeviceGuard guard(t.device());
     hcblasHandle_t h = nullptr;
     check(hcblasCreate(&h), "hcblasCreate");
-    it = handles.emplace(dev, h).first;
+    check(hcblasSetStream(h, key.stream), "hcblasSetStream");
+    it = handles.emplace(key, h).first;
   }
-  hipStream_t s = c10::hip::getCurrentHIPStream(dev).stream();
-  check(hcblasSetStream(it->second, s), "hcblasSetStream");
   return it->second;
 }
 
