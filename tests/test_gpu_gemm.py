@@ -11,6 +11,8 @@ double-double value (c = 2).
 import numpy as np
 import pytest
 
+from gemm_bounds import sample_bound
+from gemm_bounds import check_sampled
 from gpu_util import bitwise_equal, to_dev, to_host
 
 pytestmark = pytest.mark.gpu
@@ -103,20 +105,16 @@ def _rand(orc, p, n, seed):
 
 def _check_against_exact(orc, p, o, ta, tb, M, N, K, alpha, A, lda, B, ldb,
                          beta, C, ldc, got, nsamp=512):
+    """Componentwise bound c*sqrt(K)*u*(|alpha| |op(A)||op(B)| + |beta||C|)
+    against the double-double value (tests/gemm_bounds.py; c = 4 real,
+    8 complex)."""
     rng = np.random.default_rng(M * 131 + N * 7 + K)
     ns = min(nsamp, M * N)
     rows = rng.integers(0, M, ns)
     cols = rng.integers(0, N, ns)
-    ref = orc.gemm_sample_exact(o, ta, tb, M, N, K, alpha, A, lda, B, ldb,
-                                beta, C, ldc, rows, cols)
-    idx = rows + cols * ldc if o == COL else rows * ldc + cols
-    eps = np.finfo(DT[p]).eps
-    # |a|,|b|,|c| <= 1 (complex: <= sqrt 2)
-    mag = 2.0 if p in "cz" else 1.0
-    tol = 2 * (K + 2) * eps * (abs(alpha) * K * mag + abs(beta) * mag + 1)
-    err = np.abs(got[idx] - ref)
-    assert np.all(np.isfinite(got[idx]))
-    assert err.max() <= tol, (err.max(), tol)
+    u = np.finfo(DT[p]).eps
+    check_sampled(orc, o, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C,
+                  ldc, got, rows, cols, u, c=4.0 if p in "sd" else 8.0)
 
 
 SHAPES = [(1, 1, 1), (5, 3, 7), (16, 16, 16), (64, 64, 64), (65, 63, 67),
@@ -347,11 +345,11 @@ def _sampled_check(hc, handle, orc, torch, p, ta, tb, M, N, K, seed):
     got = to_host(dC[idx]).astype(np.float64)
     ref = orc.gemm_sample_exact(COL, ta, tb, M, N, K, alpha, A, ar, B, br,
                                 beta, C, M, rows, cols)
-    eps = np.finfo(dt).eps
     rel_l2 = np.linalg.norm(got - ref) / np.linalg.norm(ref)
     assert rel_l2 <= (1e-5 if p == "s" else 1e-13), rel_l2
-    tol = 2 * (K + 2) * eps * (abs(alpha) * K + abs(beta) + 1)
-    assert np.abs(got - ref).max() <= tol
+    bound = sample_bound(orc, COL, ta, tb, M, N, K, alpha, A, ar, B, br, beta,
+                         C, M, rows, cols, ref, np.finfo(dt).eps)
+    assert np.all(np.abs(got - ref) <= bound)
 
 
 @pytest.mark.parametrize("p", ["s", "d"])

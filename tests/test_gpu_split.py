@@ -19,6 +19,7 @@ import ctypes
 import numpy as np
 import pytest
 
+from gemm_bounds import check_sampled
 from gpu_util import bitwise_equal, to_dev, to_host
 
 pytestmark = pytest.mark.gpu
@@ -96,11 +97,8 @@ def test_all_cases_within_plain_bound(hc, handle, orc, p, M, N, K):
                 rng = np.random.default_rng(0)
                 rows = rng.integers(0, M, 256)
                 cols = rng.integers(0, N, 256)
-                ref = orc.gemm_sample_exact(o, ta, tb, M, N, K, 0.5, A, lda, B,
-                                            ldb, -1.5, C, ldc, rows, cols)
-                idx = rows + cols * ldc if o == COL else rows * ldc + cols
-                tol = 2 * (K + 2) * eps * (0.5 * K + 1.5 + 1)
-                assert np.max(np.abs(got[idx] - ref)) <= tol, (o, ta, tb)
+                check_sampled(orc, o, ta, tb, M, N, K, 0.5, A, lda, B, ldb,
+                              -1.5, C, ldc, got, rows, cols, eps, c=4.0)
 
 
 @pytest.mark.parametrize("p", ["s", "sc", "d", "dc"])
