@@ -268,6 +268,85 @@ def test_pointer_array_batched(hc, handle, orc, torch_cuda, p):
                              Bs[i], N, beta, Cs[i], M, got, nsamp=128)
 
 
+@pytest.mark.parametrize("p", ["s", "d", "c", "z", "h"])
+def test_pointer_array_batch_element_offsets(hc, handle, orc, torch_cuda, p):
+    """Pointer arrays whose entries point at odd element offsets inside one
+    arena (bases not 16-byte aligned, leading dimensions and the common
+    offset aligned): the kernel checks each problem's bases and takes the
+    element-wise loaders for the misaligned ones. Sizes large enough for the
+    16-byte path (every tile shape's main loop) in both trans cases."""
+    torch = torch_cuda
+    api = hc.api
+    M, N, K, b = 192, 160, 256, 6
+    dt = {"s": np.float32, "d": np.float64, "c": np.complex64,
+          "z": np.complex128, "h": np.float16}[p]
+    esz = np.dtype(dt).itemsize
+    for ta in (N_, T_):
+        lda = M if ta == N_ else K
+        sa, sb, sc = lda * (K if ta == N_ else M), K * N, M * N
+        # problem i starts at element offset off[i] of its arena (0 keeps
+        # one aligned entry beside the misaligned ones)
+        off = [0, 1, 3, 5, 2, 7][:b]
+        span_a = sum(sa + o for o in off) + 8
+        span_b = sum(sb + o for o in off) + 8
+        span_c = sum(sc + o for o in off) + 8
+        if p == "h":
+            rng = np.random.default_rng(9 + ta)
+            Ah = rng.uniform(-1, 1, span_a).astype(np.float16)
+            Bh = rng.uniform(-1, 1, span_b).astype(np.float16)
+            Ch = rng.uniform(-1, 1, span_c).astype(np.float16)
+        else:
+            Ah = _rand(orc, p, span_a, 21 + ta)
+            Bh = _rand(orc, p, span_b, 22 + ta)
+            Ch = _rand(orc, p, span_c, 23 + ta)
+        dA = to_dev(Ah.view(np.uint16) if p == "h" else Ah)
+        dB = to_dev(Bh.view(np.uint16) if p == "h" else Bh)
+        dC = to_dev(Ch.view(np.uint16) if p == "h" else Ch)
+        starts_a, starts_b, starts_c = [], [], []
+        ca = cb = cc = 0
+        for o in off:
+            starts_a.append(ca + o)
+            starts_b.append(cb + o)
+            starts_c.append(cc + o)
+            ca += sa + o
+            cb += sb + o
+            cc += sc + o
+        pa = torch.tensor([dA.data_ptr() + s_ * esz for s_ in starts_a],
+                          dtype=torch.int64, device="cuda")
+        pb = torch.tensor([dB.data_ptr() + s_ * esz for s_ in starts_b],
+                          dtype=torch.int64, device="cuda")
+        pc = torch.tensor([dC.data_ptr() + s_ * esz for s_ in starts_c],
+                          dtype=torch.int64, device="cuda")
+        alpha = 0.75 if p in "sdh" else 0.75 - 0.5j
+        beta = 0.5 if p in "sdh" else 0.5 + 0.25j
+        fn = getattr(api, f"hcblas{p.upper()}gemmBatched")
+        assert fn(handle, ta, N_, M, N, K, alpha, pa, lda, pb, K, beta, pc, M,
+                  b) == 0
+        got = to_host(dC)
+        if p == "h":
+            got = got.view(np.uint16).view(np.float16)
+        for i in range(b):
+            Ai = Ah[starts_a[i]:starts_a[i] + sa]
+            Bi = Bh[starts_b[i]:starts_b[i] + sb]
+            Ci = Ch[starts_c[i]:starts_c[i] + sc]
+            gi = got[starts_c[i]:starts_c[i] + sc]
+            if p == "h":
+                ref = orc.hgemm(COL, ta, N_, M, N, K, alpha, Ai, lda, Bi, K,
+                                beta, Ci, M).astype(np.float64)
+                ulp = np.spacing(np.abs(ref).astype(np.float16)).astype(
+                    np.float64)
+                assert np.all(np.abs(gi.astype(np.float64) - ref) <=
+                              ulp + K * 2.0 ** -23 * 2), (ta, i)
+            else:
+                _check_against_exact(orc, p, COL, ta, N_, M, N, K, alpha, Ai,
+                                     lda, Bi, K, beta, Ci, M, gi, nsamp=256)
+        # untouched outside the M x N blocks (gaps between the problems)
+        mask = np.ones(span_c, bool)
+        for st_ in starts_c:
+            mask[st_:st_ + sc] = False
+        assert bitwise_equal(got[mask], Ch[mask])
+
+
 @pytest.mark.parametrize("p", ["s", "d", "c", "z"])
 def test_strided_batched_row_major(hc, handle, orc, p):
     """Row-major strided batches (the reference silently skipped row-major
