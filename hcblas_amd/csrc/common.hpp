@@ -31,6 +31,15 @@ struct Workspace {
   size_t bytes = 0;
 };
 
+// Fixed per-GPU scratch of the allocation-free GEMV entry and the largest
+// device index it serves.
+constexpr size_t kNoAllocScratchBytes = HCBLAS_GEMV_NOALLOC_SCRATCH_BYTES;
+constexpr int kMaxDevices = 64;
+
+// Device address of the no-allocation scratch on GPU `dev` (looked up once
+// per device); nullptr if the lookup failed.
+void *scratch_address(int dev);
+
 }  // namespace hcb
 
 struct hcblasContext {
@@ -68,6 +77,8 @@ struct hcblasContext {
   hcb::Workspace adj_ws;
   // Split-K partial tiles of the plain GEMM.
   hcb::Workspace kpart_ws;
+  // This device's address of the fixed scratch of hcblas_gemv_noalloc.
+  void *noalloc_scratch = nullptr;
 };
 
 namespace hcb {

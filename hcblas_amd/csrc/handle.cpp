@@ -3,12 +3,41 @@
 // the hipBLAS stream semantics of lib/src/hcc_detail/hipblas.cpp:51-62.
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
 #include "common.hpp"
 
+// Split partials of hcblas_gemv_noalloc: one fixed array on each GPU, used
+// in place of the handle's grow-on-demand GEMV workspace so that the call
+// allocates nothing. Host code never passes this name to a kernel or a copy:
+// it uses the device address that hipGetSymbolAddress returns for the
+// current GPU (scratch_address below).
+static __device__ __align__(256) char scratch[hcb::kNoAllocScratchBytes];
+
 namespace hcb {
+
+void *scratch_address(int dev) {
+  static std::mutex mu;
+  static void *addr[kMaxDevices] = {};
+  if (dev < 0 || dev >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(mu);
+  if (addr[dev] == nullptr) {
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess) return nullptr;
+    if (cur != dev && hipSetDevice(dev) != hipSuccess) return nullptr;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) (void)hipSetDevice(cur);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    addr[dev] = p;
+  }
+  return addr[dev];
+}
 
 hipError_t ensure_workspace(Workspace &ws, size_t bytes) {
   if (ws.bytes >= bytes) return hipSuccess;
@@ -58,6 +87,13 @@ hcblasStatus_t hcblasCreate(hcblasHandle_t *handle) {
   if (h == nullptr) return HCBLAS_STATUS_ALLOC_FAILED;
   h->device = dev;
   h->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  // Device address of this GPU's no-allocation scratch (looked up once per
+  // device, with the device current).
+  h->noalloc_scratch = scratch_address(dev);
+  if (h->noalloc_scratch == nullptr) {
+    delete h;
+    return HCBLAS_STATUS_INTERNAL_ERROR;
+  }
 
   bool ok = hipMalloc(&h->d_partials, sizeof(double) * kMaxPartials) ==
                 hipSuccess &&

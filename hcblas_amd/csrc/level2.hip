@@ -22,11 +22,13 @@
 // v = y, w = x for row-major this is bit-identical to CBLAS (netlib
 // "temp = alpha*y(j); a(i,j) += x(i)*temp", row-major by argument swap).
 #include "common.hpp"
+#include "gemv_plan.hpp"
 
 namespace hcb {
 namespace {
 
 constexpr int kThreads = 256;
+static_assert(kThreads == kGemvThreads, "plan and kernels agree");
 
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef double d2v __attribute__((ext_vector_type(2)));
@@ -237,8 +239,6 @@ __global__ __launch_bounds__(kThreads) void gemv_panel_kernel(
 // against 166 / 315 us for the panel kernel. Rows of a partial last band
 // load a clamped row (the band's first) and are never stored.
 // ---------------------------------------------------------------------------
-constexpr int kBandNRows = 4;  // XR of the band-N kernel
-constexpr int kBandNCols = 8;  // UC: columns in flight
 
 template <typename T, bool NT>
 __global__ __launch_bounds__(kThreads) void gemv_bandn_kernel(
@@ -326,8 +326,6 @@ __global__ __launch_bounds__(kThreads) void gemv_bandn_kernel(
 // row range with 16-byte loads, reusing each x load for the CPW columns.
 // blockIdx.y selects a contiguous range of rows (split).
 // ---------------------------------------------------------------------------
-constexpr int kCPW = 4;
-constexpr int kColsPerBlock = 4 * kCPW;
 
 template <typename T, bool VEC, bool NT>
 __global__ __launch_bounds__(kThreads) void gemv_column_kernel(
@@ -458,7 +456,6 @@ __global__ __launch_bounds__(kThreads) void gemv_column_kernel(
 // Bands past the first full ones (R not a multiple of the band) load from a
 // clamped row (the band's first) with x = 0, so every load stays unpredicated.
 // ---------------------------------------------------------------------------
-constexpr int kBandCols = 8;
 
 template <typename T, int XR, bool NT>
 __global__ __launch_bounds__(kThreads) void gemv_band_kernel(
@@ -632,7 +629,8 @@ template <typename T>
 hcblasStatus_t run_gemv(hcblasHandle_t h, bool panel, int64_t R, int64_t Cc,
                         T alpha, const T *A, int64_t ld, const T *x,
                         int64_t incx, T beta, T *y, int64_t incy, int64_t sA,
-                        int64_t sx, int64_t sy, int64_t batch) {
+                        int64_t sx, int64_t sy, int64_t batch,
+                        void *fixed_part = nullptr, size_t fixed_bytes = 0) {
   const int64_t len_y = panel ? R : Cc;
   const int64_t len_x = panel ? Cc : R;
   if (len_y == 0 || batch == 0) return HCBLAS_STATUS_SUCCESS;
@@ -652,58 +650,28 @@ hcblasStatus_t run_gemv(hcblasHandle_t h, bool panel, int64_t R, int64_t Cc,
   const bool vec = aligned16(A) && (ld % VN == 0) && (R % VN == 0) &&
                    (!xvec || aligned16(x)) &&
                    (batch == 1 || (sA % VN == 0 && (!xvec || sx % VN == 0)));
-  const int64_t target = (int64_t)h->num_cus * 4;  // workgroups to aim for
   const bool nt = (int64_t)sizeof(T) * R * Cc * batch >= kStreamBytes;
   GemvShape s{R, Cc, ld, incx, incy, sA, sx, sy, 1};
-  // Band kernel for the column case when a band of 16 (or 8) row steps
-  // fits and the column blocks alone give the grid enough workgroups.
-  int band_xr = 0;
-  if (!panel && vec) {
-    for (int xr : {16, 8}) {
-      const int64_t bands = cdiv(R, (int64_t)xr * kThreads * VN);
-      if (R >= (int64_t)xr * kThreads * VN && bands <= 65535 &&
-          cdiv(Cc, kBandCols) * bands * batch >= target) {
-        band_xr = xr;
-        break;
-      }
-    }
-  }
-  // Band-N kernel for the row-contiguous case: whole 16 KiB bands and at
-  // least 64 columns.
-  const int64_t bandn_rows = (int64_t)kBandNRows * kThreads * VN;
-  const bool bandn = panel && vec && R >= bandn_rows && Cc >= 64 &&
-                     cdiv(R, bandn_rows) <= 65535;
-  int64_t oblocks;
-  if (bandn) {
-    oblocks = cdiv(R, bandn_rows);
-    const int64_t per_batch = std::max<int64_t>(
-        1, cdiv((int64_t)h->num_cus, oblocks * batch));
-    s.splits = (int)std::max<int64_t>(
-        1, std::min<int64_t>(cdiv(Cc, 256), per_batch));
-  } else if (panel) {
-    oblocks = cdiv(R, 64 * (int64_t)(vec ? VN : 1));
-    // at least 32 columns per wave-interleaved split
-    int64_t max_split = std::max<int64_t>(1, Cc / 128);
-    s.splits = (int)std::min<int64_t>(
-        std::max<int64_t>(1, cdiv(target, oblocks * batch)), max_split);
-  } else if (band_xr) {
-    oblocks = cdiv(Cc, kBandCols);
-    s.splits = (int)cdiv(R, (int64_t)band_xr * kThreads * VN);
-  } else {
-    oblocks = cdiv(Cc, kColsPerBlock);
-    int64_t rsteps = cdiv(R, 64 * (vec ? VN : 1));
-    int64_t max_split = std::max<int64_t>(1, rsteps / 16);
-    s.splits = (int)std::min<int64_t>(
-        std::max<int64_t>(1, cdiv(target, oblocks * batch)), max_split);
-  }
-  if (!band_xr && !bandn && s.splits > 256) s.splits = 256;
+  const GemvPlan plan =
+      plan_gemv(h->num_cus, panel, R, Cc, vec, VN, sizeof(T), batch);
+  s.splits = plan.splits;
+  const int64_t oblocks = plan.oblocks;
+  const bool bandn = plan.kernel == GemvKernel::kBandN;
+  const int band_xr = plan.kernel == GemvKernel::kBand16  ? 16
+                      : plan.kernel == GemvKernel::kBand8 ? 8
+                                                          : 0;
   T *part = nullptr;
   if (s.splits > 1) {
-    size_t pbytes = sizeof(T) * (size_t)s.splits * (size_t)len_y *
-                    (size_t)std::min<int64_t>(batch, 65535);
-    if (ensure_workspace(h->gemv_partials, pbytes) != hipSuccess)
-      return HCBLAS_STATUS_ALLOC_FAILED;
-    part = static_cast<T *>(h->gemv_partials.ptr);
+    if (fixed_part) {
+      // Allocation-free entry: the same split (so the same summation order
+      // and results as the workspace path) or an error, never fewer splits.
+      if (plan.partial_bytes > fixed_bytes) return HCBLAS_STATUS_INVALID_VALUE;
+      part = static_cast<T *>(fixed_part);
+    } else {
+      if (ensure_workspace(h->gemv_partials, plan.partial_bytes) != hipSuccess)
+        return HCBLAS_STATUS_ALLOC_FAILED;
+      part = static_cast<T *>(h->gemv_partials.ptr);
+    }
   }
   for (int64_t b0 = 0; b0 < batch; b0 += 65535) {
     int64_t nb = std::min<int64_t>(65535, batch - b0);
@@ -776,7 +744,7 @@ hcblasStatus_t gemv_entry(hcblasHandle_t h, hcblasOrder_t order,
                           int64_t incx, const T *beta, T *y, int64_t yOffset,
                           int64_t incy, int64_t batch, bool ref_batch_strides,
                           int64_t sA_in = 0, int64_t sx_in = 0,
-                          int64_t sy_in = 0) {
+                          int64_t sy_in = 0, bool noalloc = false) {
   if (!handle_ok(h)) return HCBLAS_STATUS_NOT_INITIALIZED;
   if (M < 0 || N < 0 || incx == 0 || incy == 0 || batch < 0)
     return HCBLAS_STATUS_INVALID_VALUE;
@@ -804,7 +772,8 @@ hcblasStatus_t gemv_entry(hcblasHandle_t h, hcblasOrder_t order,
   const bool panel = col == notrans;
   return run_gemv<T>(h, panel, R, Cc, *alpha, A ? A + aOffset : nullptr, lda,
                      x ? x + xOffset : nullptr, incx, *beta, y + yOffset, incy,
-                     sA, sx, sy, batch);
+                     sA, sx, sy, batch, noalloc ? h->noalloc_scratch : nullptr,
+                     noalloc ? kNoAllocScratchBytes : 0);
 }
 
 template <typename T>
@@ -1037,6 +1006,32 @@ hcblasStatus_t hcblas_dger_batched(
   return ger_entry<double>(handle, order, M, N, alpha, X, xOffset, incX, Y,
                            yOffset, incY, A, aOffset, lda, batchSize, false,
                            A_batchOffset, X_batchOffset, Y_batchOffset);
+}
+
+hcblasStatus_t hcblas_gemv_noalloc(hcblasHandle_t handle,
+                                   hcblasDatatype_t dtype, hcblasOrder_t order,
+                                   hcblasOperation_t type, int M, int N,
+                                   const void *alpha, const void *A,
+                                   int64_t aOffset, int64_t lda, const void *X,
+                                   int64_t xOffset, int64_t incX,
+                                   const void *beta, void *Y, int64_t yOffset,
+                                   int64_t incY) {
+  if (!handle_ok(handle)) return HCBLAS_STATUS_NOT_INITIALIZED;
+  if (dtype == HCBLAS_R_32F)
+    return gemv_entry<float>(
+        handle, order, type, M, N, static_cast<const float *>(alpha),
+        static_cast<const float *>(A), aOffset, lda,
+        static_cast<const float *>(X), xOffset, incX,
+        static_cast<const float *>(beta), static_cast<float *>(Y), yOffset,
+        incY, 1, false, 0, 0, 0, true);
+  if (dtype == HCBLAS_R_64F)
+    return gemv_entry<double>(
+        handle, order, type, M, N, static_cast<const double *>(alpha),
+        static_cast<const double *>(A), aOffset, lda,
+        static_cast<const double *>(X), xOffset, incX,
+        static_cast<const double *>(beta), static_cast<double *>(Y), yOffset,
+        incY, 1, false, 0, 0, 0, true);
+  return HCBLAS_STATUS_INVALID_VALUE;
 }
 
 hcblasStatus_t hcblas_sger_backward(hcblasHandle_t handle,

@@ -491,6 +491,28 @@ hcblasStatus_t hcblas_dger_batched(
     int64_t Y_batchOffset, int64_t incY, double *A, int64_t aOffset,
     int64_t A_batchOffset, int64_t lda, int batchSize);
 
+/* Allocation-free GEMV (no counterpart in the reference). For callers that
+ * cannot allocate device memory at call time: code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc. Same arguments and
+ * results (bit for bit) as hcblas_sgemv (dtype HCBLAS_R_32F) or hcblas_dgemv
+ * (HCBLAS_R_64F), but the split partials of the reduction live in a fixed
+ * device array of HCBLAS_GEMV_NOALLOC_SCRATCH_BYTES on each GPU instead of
+ * the handle's grow-on-demand workspace, and the call makes no allocation.
+ * A problem whose partials do not fit returns HCBLAS_STATUS_INVALID_VALUE
+ * (16 MiB holds every single GEMV up to 16 GiB of A, e.g. 16384 x 16384 in
+ * both types and both transposes). Calls that overlap on one GPU (other
+ * streams, other handles, other threads) share the array and race; the
+ * array keeps the last call's partials, none of which needs clearing. */
+#define HCBLAS_GEMV_NOALLOC_SCRATCH_BYTES ((size_t)16 << 20)
+hcblasStatus_t hcblas_gemv_noalloc(hcblasHandle_t handle,
+                                   hcblasDatatype_t dtype, hcblasOrder_t order,
+                                   hcblasOperation_t type, int M, int N,
+                                   const void *alpha, const void *A,
+                                   int64_t aOffset, int64_t lda, const void *X,
+                                   int64_t xOffset, int64_t incX,
+                                   const void *beta, void *Y, int64_t yOffset,
+                                   int64_t incY);
+
 /* ------------------------------------------------------------------------ */
 /* Level 3 (column-major C API, hcblas.h:662-790)                           */
 /* Aarray / Barray / Carray are DEVICE arrays of device pointers             */
