@@ -14,8 +14,11 @@ compulsory traffic (each operand read once, outputs written once, y read only
 when beta != 0). The JSON line also carries
   roofline      -- the dominant kernel (largest share of the step) measured
                    with HIP events on the stream it is launched on;
-  cpu_baseline  -- the oracle's C restatement timed on the host (1 core) on a
-                   bounded sample of the same workload;
+  cpu_baseline  -- CBLAS (OpenBLAS through scipy, the job's host cores) on
+                   the full bandwidth set with the GPU's own inputs (the
+                   oracle's single-thread C restatement on a bounded sample
+                   when no BLAS library is importable); cpu_gemm gives the
+                   host SGEMM / DGEMM rates of the GEMM configurations;
   ops           -- per-routine GB/s;
   gemm          -- the GEMM half of the metric: plain SGEMM/DGEMM 8192^3 and
                    split / Ozaki forms in TFLOP/s with max/mean relative error
@@ -43,7 +46,77 @@ def parse():
     ap.add_argument("--no-gemm", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--gemm-n", type=int, default=8192)
+    ap.add_argument("--case", default=None, choices=sorted(CASES),
+                    help="run only this GEMM case `--reps` times (for "
+                         "profiler counter passes) and print its median ms")
+    ap.add_argument("--reps", type=int, default=5)
     return ap.parse_args()
+
+
+# Single GEMM cases of the GEMM configurations, for profiler passes:
+# name -> (dtype tag, size, entry, row-major?)
+CASES = {
+    "sgemm_8192": ("s", 8192, "hcblasSgemm", False),
+    "dgemm_8192": ("d", 8192, "hcblasDgemm", False),
+    "sgemm_rowmajor_1024": ("s", 1024, "hcblas_sgemm", True),
+    "sgemm_split_bf16_8192": ("s", 8192, "hcblas_sgemm_split_bf16", False),
+    "sgemm_ozaki_crt_8192": ("s", 8192, "hcblas_sgemm_ozaki_crt", False),
+    "dgemm_ozaki_crt_8192": ("d", 8192, "hcblas_dgemm_ozaki_crt", False),
+    "grouped_4096": (None, 0, None, False),
+    "adjoint_bf16": (None, 0, None, False),
+}
+
+
+def run_case(hc, h, torch, name, reps):
+    """Runs one entry of CASES `reps` times after one warm-up call; returns
+    the median milliseconds per call (HIP events on the handle's stream)."""
+    api = hc.api
+    tag, n, entry, row = CASES[name]
+    g = torch.Generator(device="cuda").manual_seed(7)
+    if name == "grouped_4096":
+        problems, meta, arenas = grouped_problem_table(torch, hc, 4096)
+        plan = hc.GroupedPlan(h, problems)
+        if plan.autotune(3) != 0:
+            raise RuntimeError("autotune failed")
+        fn = plan.run
+    elif name == "adjoint_bf16":
+        from hcblas_amd import torch_ext
+        b, M, N, K = 256, 256, 1024, 512
+        A = (torch.rand(b, M, K, generator=g, device="cuda") * 2 - 1).bfloat16()
+        B = (torch.rand(b, K, N, generator=g, device="cuda") * 2 - 1).bfloat16()
+        W = torch.rand(b, M, N, generator=g, device="cuda") * 2 - 1
+        A.requires_grad_(True)
+        B.requires_grad_(True)
+
+        def fn():
+            A.grad = None
+            B.grad = None
+            (torch_ext.gemm_batched(A, B) * W).sum().backward()
+            return 0
+    else:
+        dt = torch.float32 if tag == "s" else torch.float64
+        A = torch.rand(n * n, generator=g, device="cuda", dtype=dt) * 2 - 1
+        B = torch.rand(n * n, generator=g, device="cuda", dtype=dt) * 2 - 1
+        C = torch.zeros(n * n, device="cuda", dtype=dt)
+        f = getattr(api, entry)
+        if entry.startswith("hcblas_"):
+            order = 0 if row else 1
+            fn = lambda: f(h, order, 0, 0, n, n, n, 1.0, A, n, B, n, 0.0, C, n,
+                           0, 0, 0)
+        else:
+            fn = lambda: f(h, 0, 0, n, n, n, 1.0, A, n, B, n, 0.0, C, n)
+    if fn() != 0:
+        raise RuntimeError(f"{name} failed")
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ts = []
+    for _ in range(reps):
+        s.record()
+        fn()
+        e.record()
+        torch.cuda.synchronize()
+        ts.append(s.elapsed_time(e))
+    return sorted(ts)[len(ts) // 2]
 
 
 def bandwidth_ops(hc, h, torch):
@@ -88,14 +161,17 @@ def bandwidth_ops(hc, h, torch):
         ("sger", r1_s, lambda: chk(api.hcblasSger(h, n_r1, n_r1, 1.5, gxs, 1, gys, 1, G_s, n_r1))),
         ("dger", r1_d, lambda: chk(api.hcblasDger(h, n_r1, n_r1, 1.5, gxd, 1, gyd, 1, G_d, n_r1))),
         ("saxpy", 3 * 4 * n_s, lambda: chk(api.hcblasSaxpy(h, n_s, 1.5, vx, 1, vy, 1))),
-        ("sscal", 2 * 4 * n_s, lambda: chk(api.hcblasSscal(h, n_s, 1.0, vy, 1))),
+        ("sscal", 2 * 4 * n_s, lambda: chk(api.hcblasSscal(h, n_s, -1.0, vy, 1))),
         ("scopy", 2 * 4 * n_s, lambda: chk(api.hcblasScopy(h, n_s, vx, 1, vz, 1))),
         ("sdot", 2 * 4 * n_s, lambda: chk(api.hcblasSdot(h, n_s, vx, 1, vy, 1, None)[0])),
         ("sasum", 4 * n_s, lambda: chk(api.hcblasSasum(h, n_s, vx, 1, None)[0])),
         ("ddot", 2 * 8 * n_d, lambda: chk(api.hcblasDdot(h, n_d, dx, 1, dy, 1, None)[0])),
         ("dasum", 8 * n_d, lambda: chk(api.hcblasDasum(h, n_d, dx, 1, None)[0])),
     ]
-    return ops
+    tensors = {"A_s": A_s, "A_d": A_d, "xs": xs, "ys": ys, "xd": xd, "yd": yd,
+               "G_s": G_s, "G_d": G_d, "gxs": gxs, "gys": gys, "gxd": gxd,
+               "gyd": gyd, "vx": vx, "vy": vy, "vz": vz, "dx": dx, "dy": dy}
+    return ops, tensors
 
 
 def measured_traffic(op):
@@ -155,7 +231,7 @@ def gemm_section(hc, h, torch, oracle, n, reps=20, nc=4096):
     sign * 2^u (u ~ U[-20, 20]); C / Z GEMM at nc^3, NN, uniform. Every
     result carries its median time over `reps` calls, TFLOP/s at the
     plain-equivalent 2MNK (8MNK complex), and max / mean relative error vs
-    the double-double oracle on 256 sampled entries; split results also
+    the double-double oracle on 4096 sampled entries; split results also
     carry their speed ratio to the plain kernel on the same inputs."""
     import numpy as np
     api = hc.api
@@ -194,14 +270,22 @@ def gemm_section(hc, h, torch, oracle, n, reps=20, nc=4096):
         ref = oracle.gemm_sample_exact(1, 1, 0, ns, ns, m, 1.0, a_rows.reshape(-1), m,
                                        b_cols.reshape(-1), m, 0.0,
                                        np.zeros(1, a_rows.dtype), ns, diag, diag)
-        rel = np.abs(got.astype(ref.dtype) - ref) / np.maximum(np.abs(ref), 1e-300)
-        return float(rel.max()), float(rel.mean())
+        absp = oracle.gemm_sample_exact(
+            1, 1, 0, ns, ns, m, 1.0, np.abs(a_rows).astype(np.float64).reshape(-1), m,
+            np.abs(b_cols).astype(np.float64).reshape(-1), m, 0.0, np.zeros(1), ns,
+            diag, diag)
+        err = np.abs(got.astype(ref.dtype) - ref)
+        rel = err / np.maximum(np.abs(ref), 1e-300)
+        nrm = err / absp
+        return (float(rel.max()), float(rel.mean()), float(nrm.max()),
+                float(nrm.mean()))
 
     def result(ms, flop, peak_key, err):
         tf = flop / (ms * 1e-3) / 1e12
         return {"tflops": round(tf, 2), "ms": round(ms, 3),
                 "peak_frac": round(tf / PEAK_TFLOPS[peak_key], 4),
-                "max_rel_err": err[0], "mean_rel_err": err[1]}
+                "max_rel_err": err[0], "mean_rel_err": err[1],
+                "max_err_vs_absprod": err[2], "mean_err_vs_absprod": err[3]}
 
     def gen(m, dt, wide):
         if not wide:
@@ -445,6 +529,10 @@ def grouped_section(hc, h, torch, count=4096, reps=10):
     plan.destroy()
     bound_ms = (flops32 / (PEAK_TFLOPS["fp32"] * 1e12) +
                 flops16 / (PEAK_TFLOPS["bf16"] * 1e12)) * 1e3
+    # compulsory HBM traffic: A and B read once, C written once (beta = 0)
+    hbm_bytes = sum((m * k + k * n + m * n) * (4 if d == 0 else 2)
+                    for (d, o, m, n, k, *_) in meta)
+    hbm_ms = hbm_bytes / (HBM_PEAK_GBS * 1e9) * 1e3
     tot = flops32 + flops16
     return {"problems": count, "flop": tot, "fp32_flop": flops32,
             "grouped_ms": round(ms_g, 4), "grouped_tflops": round(tot / ms_g / 1e9, 2),
@@ -452,7 +540,9 @@ def grouped_section(hc, h, torch, count=4096, reps=10):
             "plain_launches_tflops": round(tot / ms_p / 1e9, 2),
             "speedup": round(ms_p / ms_g, 2),
             "mfma_bound_ms": round(bound_ms, 4),
-            "roofline_frac": round(bound_ms / ms_g, 4), "plan": info}
+            "hbm_bytes": hbm_bytes, "hbm_bound_ms": round(hbm_ms, 4),
+            "roofline_frac": round(max(bound_ms, hbm_ms) / ms_g, 4),
+            "plan": info}
 
 
 def adjoint_section(torch, reps=5):
@@ -511,9 +601,142 @@ def adjoint_section(torch, reps=5):
     return out
 
 
-def cpu_baseline(oracle):
-    """The oracle's C restatement (single thread) on a bounded sample of the
-    bandwidth set: gemv/ger at 8192^2, level 1 at n = 2^25 / 2^24."""
+def cpu_threads():
+    """Host threads for the CPU baseline: the cores this process may run on,
+    capped by OMP_NUM_THREADS when the environment sets it (the share of a
+    shared host that belongs to this job)."""
+    n = len(os.sched_getaffinity(0))
+    try:
+        cap = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        cap = 0
+    return min(n, cap) if cap > 0 else n
+
+
+def host_copies(tensors):
+    """Host (numpy) copies of the bandwidth set's device inputs, so that the
+    CPU baseline runs on the same seeded values as the GPU."""
+    return {k: t.cpu().numpy() for k, t in tensors.items()}
+
+
+def cpu_baseline_blas(host, reps=5):
+    """CBLAS (OpenBLAS through scipy.linalg.blas, all the job's host cores)
+    on the full bandwidth set with the GPU's inputs: 1 warm-up, then the
+    median of `reps` runs per routine; value = total algorithmic bytes /
+    sum of the medians. None when scipy's BLAS is not importable."""
+    try:
+        import numpy as np
+        import threadpoolctl
+        from scipy.linalg import blas
+        import scipy
+    except ImportError:
+        return None
+    threads = cpu_threads()
+    info = [i for i in threadpoolctl.threadpool_info()
+            if i.get("user_api") == "blas" and "scipy.libs" in i.get("filepath", "")]
+    lib = (f"{info[0]['internal_api']} {info[0]['version']}" if info
+           else "unknown BLAS")
+    n_mv, n_r1 = 16384, 8192
+    n_s, n_d = host["vx"].size, host["dx"].size
+
+    def colmajor(flat, m, n):  # m x n column-major view, Fortran-contiguous
+        return flat.reshape(n, m).T
+
+    A_s = colmajor(host["A_s"], n_mv, n_mv)
+    A_d = colmajor(host["A_d"], n_mv, n_mv)
+    G_s = colmajor(host["G_s"], n_r1, n_r1)
+    G_d = colmajor(host["G_d"], n_r1, n_r1)
+    xs, ys, xd, yd = host["xs"], host["ys"], host["xd"], host["yd"]
+    gxs, gys, gxd, gyd = host["gxs"], host["gys"], host["gxd"], host["gyd"]
+    vx, vy, vz = host["vx"], host["vy"], host["vz"]
+    dx, dy = host["dx"], host["dy"]
+    mv_s = 4 * (n_mv * n_mv + 3 * n_mv)
+    mv_d = 8 * (n_mv * n_mv + 3 * n_mv)
+    r1_s = 4 * (2 * n_r1 * n_r1 + 2 * n_r1)
+    r1_d = 8 * (2 * n_r1 * n_r1 + 2 * n_r1)
+    jobs = [
+        ("sgemv_n", mv_s, lambda: blas.sgemv(1.5, A_s, xs, 0.5, ys, trans=0, overwrite_y=1)),
+        ("sgemv_t", mv_s, lambda: blas.sgemv(1.5, A_s, xs, 0.5, ys, trans=1, overwrite_y=1)),
+        ("dgemv_n", mv_d, lambda: blas.dgemv(1.5, A_d, xd, 0.5, yd, trans=0, overwrite_y=1)),
+        ("dgemv_t", mv_d, lambda: blas.dgemv(1.5, A_d, xd, 0.5, yd, trans=1, overwrite_y=1)),
+        ("sger", r1_s, lambda: blas.sger(1.5, gxs, gys, a=G_s, overwrite_a=1)),
+        ("dger", r1_d, lambda: blas.dger(1.5, gxd, gyd, a=G_d, overwrite_a=1)),
+        ("saxpy", 12 * n_s, lambda: blas.saxpy(vx, vy, a=1.5)),
+        ("sscal", 8 * n_s, lambda: blas.sscal(-1.0, vy)),
+        ("scopy", 8 * n_s, lambda: blas.scopy(vx, vz)),
+        ("sdot", 8 * n_s, lambda: blas.sdot(vx, vy)),
+        ("sasum", 4 * n_s, lambda: blas.sasum(vx)),
+        ("ddot", 16 * n_d, lambda: blas.ddot(dx, dy)),
+        ("dasum", 8 * n_d, lambda: blas.dasum(dx)),
+    ]
+    per_op = {}
+    total_b, total_t = 0, 0.0
+    with threadpoolctl.threadpool_limits(limits=threads, user_api="blas"):
+        for name, b, fn in jobs:
+            fn()
+            ts = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                fn()
+                ts.append(time.perf_counter() - t0)
+            t = sorted(ts)[len(ts) // 2]
+            per_op[name] = round(b / t / 1e9, 2)
+            total_b += b
+            total_t += t
+    del np
+    return {"value": round(total_b / total_t / 1e9, 3), "unit": "GB/s",
+            "cores": threads, "kind": "reference",
+            "library": f"{lib} (scipy {scipy.__version__} scipy.linalg.blas), "
+                       "the CBLAS the reference's tests compare against",
+            "sample": "the full bandwidth set on the GPU run's own inputs: "
+                      "S/DGEMV 16384^2 N+T, S/DGER 8192^2, SAXPY/SSCAL/SCOPY/"
+                      "SDOT/SASUM n=2^28, DDOT/DASUM n=2^27; median of "
+                      f"{reps} after 1 warm-up per routine",
+            "seconds": round(total_t, 3), "ops": per_op}
+
+
+def cpu_gemm_blas(n_small=1024, n_big=8192, reps=5):
+    """CBLAS SGEMM / DGEMM on the host (OpenBLAS through scipy, all the job's
+    host cores): the 1024^3 row-major NN SGEMM configuration and the plain
+    8192^3 NN S/DGEMM of the precision configuration, uniform[-1,1] inputs;
+    1 warm-up, median of `reps` (2 for 8192^3)."""
+    try:
+        import numpy as np
+        import threadpoolctl
+        from scipy.linalg import blas
+    except ImportError:
+        return None
+    threads = cpu_threads()
+    rng = np.random.default_rng(0)
+    out = {"cores": threads}
+    with threadpoolctl.threadpool_limits(limits=threads, user_api="blas"):
+        for key, n, dt, fn, r in (("sgemm_rowmajor_1024", n_small, np.float32,
+                                   blas.sgemm, reps),
+                                  ("sgemm_8192", n_big, np.float32, blas.sgemm, 2),
+                                  ("dgemm_8192", n_big, np.float64, blas.dgemm, 2)):
+            A = rng.uniform(-1, 1, (n, n)).astype(dt)  # row-major M x K
+            B = rng.uniform(-1, 1, (n, n)).astype(dt)  # row-major K x N
+            # row-major C = A B is column-major C^T = B^T A^T; .T of a
+            # C-ordered array is the Fortran-ordered transpose (no copy)
+            call = (lambda A=A, B=B, fn=fn: fn(1.0, B.T, A.T))
+            call()
+            ts = []
+            for _ in range(r):
+                t0 = time.perf_counter()
+                call()
+                ts.append(time.perf_counter() - t0)
+            t = sorted(ts)[len(ts) // 2]
+            out[key] = {"gflops": round(2.0 * n ** 3 / t / 1e9, 1),
+                        "ms": round(t * 1e3, 3)}
+            del A, B
+    del np
+    return out
+
+
+def cpu_baseline_port(oracle):
+    """Fallback when no BLAS library is importable: the oracle's C
+    restatement (single thread) on a bounded sample of the bandwidth set:
+    gemv/ger at 8192^2, level 1 at n = 2^25 / 2^24."""
     import numpy as np
     n_mv, n_s, n_d = 8192, 1 << 25, 1 << 24
     A_s = oracle.uniform_f32(n_mv * n_mv, 1)
@@ -572,8 +795,17 @@ def main():
     torch.cuda.set_device(0)
     h = hc.Handle()
     h.set_stream(torch.cuda.current_stream())
-    ops = bandwidth_ops(hc, h, torch)
+    if args.case:
+        ms = run_case(hc, h, torch, args.case, args.reps)
+        h.destroy()
+        print(json.dumps({"case": args.case, "reps": args.reps,
+                          "median_ms": round(ms, 4)}))
+        return
+    ops, tensors = bandwidth_ops(hc, h, torch)
     step_ms, per_op, wall = time_bandwidth(torch, ops, args.steps, args.warmup)
+    # the CPU baseline runs later on host copies of the same inputs
+    host = None if args.no_cpu else host_copies(tensors)
+    del tensors
     total_bytes = sum(b for _, b, _ in ops)
     value = total_bytes / (step_ms * 1e-3) / 1e9
     op_gbs = {name: round(b / (t * 1e-3) / 1e9, 1) for (name, b, _), t in zip(ops, per_op)}
@@ -608,8 +840,13 @@ def main():
         result["grouped"] = grouped_section(hc, h, torch)
         result["adjoint"] = adjoint_section(torch)
     if not args.no_cpu:
-        import oracle
-        result["cpu_baseline"] = cpu_baseline(oracle)
+        base = cpu_baseline_blas(host)
+        del host
+        if base is None:
+            import oracle
+            base = cpu_baseline_port(oracle)
+        result["cpu_baseline"] = base
+        result["cpu_gemm"] = cpu_gemm_blas()
     h.destroy()
     print(json.dumps(result))
 
