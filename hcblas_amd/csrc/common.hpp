@@ -18,13 +18,11 @@ namespace hcb {
 
 constexpr uint32_t kHandleMagic = 0x48434d49u;  // "HCMI"
 
-// Reduction scratch (dot / asum): at most kMaxReduceBlocks workgroup
-// partials, reduced in groups of kReduceGroup, then across groups.
+// Largest grid of the level-1 elementwise kernels (they grid-stride beyond).
 constexpr int kMaxReduceBlocks = 1 << 21;
-constexpr int kReduceGroup = 256;
-constexpr int kMaxReduceGroups = kMaxReduceBlocks / kReduceGroup;
-constexpr size_t kMaxPartials = (size_t)kMaxReduceBlocks + kMaxReduceGroups;
-constexpr size_t kTicketCount = 1 + kMaxReduceGroups;
+// Reduction scratch (dot / asum): one FP64 partial per workgroup of the
+// bounded reduction grid.
+constexpr int kReducePartials = 256;
 
 struct Workspace {
   void *ptr = nullptr;
@@ -51,15 +49,17 @@ struct hcblasContext {
   int ozaki_slices = 8;
   int ozaki_moduli = 0;  // 0: chosen per K (see gemm_split.hip)
 
-  // dot / asum: per-workgroup partials, a ticket counter that the last
-  // workgroup resets to zero, the device result and a pinned host mirror.
+  // dot / asum: per-workgroup partials.
   double *d_partials = nullptr;
-  unsigned int *d_ticket = nullptr;
-  // Pinned host word of the host-result reductions (dot / asum) and its
-  // device address: the reduction's last workgroup writes the result there
-  // directly, so a call needs no device-to-host copy after the kernel.
+  // Pinned, host-coherent block of the host-result reductions (dot / asum)
+  // and its device address: h_result[0] holds the result and, as a 64-bit
+  // word, h_result[1] the sequence number of the call that wrote it. The
+  // reduction's last workgroup writes the result, then (system-scope
+  // release) the sequence number; the host returns as soon as it reads its
+  // call's number there, without waiting for the kernel's completion signal.
   double *h_result = nullptr;
   double *h_result_dev = nullptr;
+  uint64_t result_seq = 0;
 
   // gemv split-reduction partials and per-row-block tickets (grow-only; the
   // ticket region is zeroed when it is (re)allocated and every launch leaves

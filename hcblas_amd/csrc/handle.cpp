@@ -95,20 +95,17 @@ hcblasStatus_t hcblasCreate(hcblasHandle_t *handle) {
     return HCBLAS_STATUS_INTERNAL_ERROR;
   }
 
-  bool ok = hipMalloc(&h->d_partials, sizeof(double) * kMaxPartials) ==
+  bool ok = hipMalloc(&h->d_partials, sizeof(double) * kReducePartials) ==
                 hipSuccess &&
-            hipMalloc(&h->d_ticket, sizeof(unsigned int) * kTicketCount) ==
-                hipSuccess &&
-            hipHostMalloc(&h->h_result, 64, hipHostMallocDefault) ==
+            hipHostMalloc(&h->h_result, 64,
+                          hipHostMallocMapped | hipHostMallocCoherent) ==
                 hipSuccess &&
             hipHostGetDevicePointer(reinterpret_cast<void **>(&h->h_result_dev),
                                     h->h_result, 0) == hipSuccess &&
-            hipMemset(h->d_ticket, 0, sizeof(unsigned int) * kTicketCount) ==
-                hipSuccess &&
             hipDeviceSynchronize() == hipSuccess;
+  if (ok) std::memset(h->h_result, 0, 64);
   if (!ok) {
     if (h->d_partials) (void)hipFree(h->d_partials);
-    if (h->d_ticket) (void)hipFree(h->d_ticket);
     if (h->h_result) (void)hipHostFree(h->h_result);
     delete h;
     return HCBLAS_STATUS_ALLOC_FAILED;
@@ -126,7 +123,6 @@ hcblasStatus_t hcblasDestroy(hcblasHandle_t *handle) {
   (void)hipStreamSynchronize(h->stream);
   h->magic = 0;
   (void)hipFree(h->d_partials);
-  (void)hipFree(h->d_ticket);
   (void)hipHostFree(h->h_result);
   if (h->gemv_partials.ptr) (void)hipFree(h->gemv_partials.ptr);
   if (h->gemv_tickets.ptr) (void)hipFree(h->gemv_tickets.ptr);

@@ -11,11 +11,10 @@
 // take the vector path: one 16-byte vector per thread over a full grid
 // (measured on MI355X to stream faster than grid-stride loops), with
 // non-temporal loads / stores once the operands exceed the caches. Everything
-// else takes a scalar path that honours BLAS increments. dot / asum are one
-// launch: each workgroup reduces to an FP64 partial; the last workgroup of
-// every group of 256 (ticket counter) sums that group in index order, and the
-// last group finisher sums the group partials in index order, so the result
-// is deterministic and no host round trip is needed for the reduction.
+// else takes a scalar path that honours BLAS increments. dot / asum: each of
+// at most 256 workgroups reduces to an FP64 partial, and a one-workgroup
+// launch on the same stream sums the partials in index order (deterministic)
+// into the handle's pinned host block, whose sequence word the host polls.
 #include "common.hpp"
 
 namespace hcb {
@@ -25,11 +24,13 @@ constexpr int kThreads = 256;
 // Operands larger than this (bytes) bypass the caches (non-temporal).
 constexpr int64_t kStreamBytes = 64ll << 20;
 // Vector-path reductions: loads in flight per thread and maximum grid. A
-// sweep on MI355X (sdot, 2^28) measured 7.0 TB/s at 256 workgroups x 4
-// vectors in flight, falling to 4.6 TB/s at 2048 and 1.8 TB/s at 16384
-// workgroups (each workgroup's ticket release is an L2 write-back).
+// sweep on MI355X (sdot, 2^28) measured 7.2 TB/s at 256 workgroups x 4
+// vectors in flight and 6.6 TB/s at 512. The finish is a second launch:
+// an in-kernel "last workgroup sums" ticket costs an agent-scope release
+// (an L2 write-back on this multi-XCD part) per workgroup, 9 us of the
+// 2^28 SDOT (307 against 298 us).
 constexpr int kReduceUnroll = 4;
-constexpr int kReduceGrid = 256;
+constexpr int kReduceGrid = kReducePartials;
 
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef double d2v __attribute__((ext_vector_type(2)));
@@ -200,6 +201,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // Block sum of one double per thread, in a fixed order (valid on thread 0).
+template <int TH = kThreads>
 __device__ __forceinline__ double block_sum(double v, double *lds) {
   v = wave_sum(v);
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
@@ -209,89 +211,52 @@ __device__ __forceinline__ double block_sum(double v, double *lds) {
   double r = 0.0;
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int i = 0; i < kThreads / 64; ++i) r += lds[i];
+    for (int i = 0; i < TH / 64; ++i) r += lds[i];
   }
   return r;
 }
 
-// Publishes this workgroup's partial and runs the deterministic two-level
-// finish: the last workgroup of each group of kReduceGroup sums the group's
-// partials in index order into a group partial; the last group finisher sums
-// the group partials in index order and writes the result as T. Every ticket
-// it takes is returned to zero.
+// Final sum of the reduction kernels' partials (one per workgroup, at most
+// kReduceGrid, written with plain stores; this launch follows on the same
+// stream, so the kernel boundary makes them visible). The partials are added
+// in workgroup order, so the result does not depend on timing. The result
+// goes straight into the handle's pinned host block, then this call's
+// sequence number, which the host waits for (release at system scope orders
+// the two stores for the host).
 template <typename T>
-__device__ void tree_finish(double partial, double *partials,
-                            unsigned int *tickets, T *result, double *lds) {
-  __shared__ bool last;
-  const unsigned int nb = gridDim.x, bid = blockIdx.x;
-  const unsigned int ngroups = (nb + kReduceGroup - 1) / kReduceGroup;
-  const unsigned int g = bid / kReduceGroup;
-  const unsigned int g0 = g * kReduceGroup;
-  const unsigned int gsize = nb - g0 < (unsigned)kReduceGroup
-                                 ? nb - g0
-                                 : (unsigned)kReduceGroup;
-  double *gpart = partials + kMaxReduceBlocks;
-  if (threadIdx.x == 0) {
-    __hip_atomic_store(&partials[bid], partial, __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-    unsigned int t = __hip_atomic_fetch_add(&tickets[1 + g], 1u,
-                                            __ATOMIC_ACQ_REL,
-                                            __HIP_MEMORY_SCOPE_AGENT);
-    last = (t == gsize - 1);
-  }
-  __syncthreads();
-  if (!last) return;
-  // Group finisher (kReduceGroup == kThreads: one partial per thread).
-  double s = threadIdx.x < gsize
-                 ? __hip_atomic_load(&partials[g0 + threadIdx.x],
-                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                 : 0.0;
-  double gs = block_sum(s, lds);
-  if (threadIdx.x == 0) {
-    __hip_atomic_store(&tickets[1 + g], 0u, __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(&gpart[g], gs, __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-    unsigned int t = __hip_atomic_fetch_add(&tickets[0], 1u, __ATOMIC_ACQ_REL,
-                                            __HIP_MEMORY_SCOPE_AGENT);
-    last = (t == ngroups - 1);
-  }
-  __syncthreads();
-  if (!last) return;
-  double f = 0.0;
-  for (unsigned int i = threadIdx.x; i < ngroups; i += kThreads)
-    f += __hip_atomic_load(&gpart[i], __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-  double tot = block_sum(f, lds);
+__global__ __launch_bounds__(kThreads) void reduce_finish_kernel(
+    int nparts, const double *__restrict__ partials, T *result,
+    unsigned long long *done, unsigned long long seq) {
+  __shared__ double lds[kThreads / 64];
+  const double v = (int)threadIdx.x < nparts ? partials[threadIdx.x] : 0.0;
+  const double tot = block_sum(v, lds);
   if (threadIdx.x == 0) {
     *result = (T)tot;
-    __hip_atomic_store(&tickets[0], 0u, __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(done, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 
 // Vector-path reduction: grid-stride over 16-byte vectors of x (and y) with
 // U vectors in flight per thread; the scalar tail (< one vector) is added by
-// block 0. The grid is kept to a few thousand workgroups: every workgroup
-// takes one agent-scope ticket, whose release is an L2 write-back on this
-// multi-XCD part, so a full grid of ~10^5 workgroups would serialise on them.
-template <typename T, typename Red, bool NT, int U>
-__global__ __launch_bounds__(kThreads) void reduce_vec_kernel(
+// block 0. Each workgroup writes one FP64 partial; reduce_finish_kernel sums
+// them.
+template <typename T, typename Red, bool NT, int U, int TH = kThreads>
+__global__ __launch_bounds__(TH) void reduce_vec_kernel(
     int64_t n, const T *__restrict__ x, const T *__restrict__ y,
-    double *partials, unsigned int *tickets, T *result, Red red) {
+    double *partials, Red red) {
   using V = typename V16<T>::type;
   constexpr int VN = V16<T>::n;
-  __shared__ double lds[kThreads / 64];
+  __shared__ double lds[TH / 64];
   const V *xv = reinterpret_cast<const V *>(x);
   const V *yv = reinterpret_cast<const V *>(y);
   const int64_t nvec = n / VN;
-  const int64_t stride = (int64_t)gridDim.x * kThreads;
+  const int64_t stride = (int64_t)gridDim.x * TH;
   T acc[U][VN];
 #pragma unroll
   for (int u = 0; u < U; ++u)
 #pragma unroll
     for (int e = 0; e < VN; ++e) acc[u][e] = T(0);
-  int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  int64_t i = (int64_t)blockIdx.x * TH + threadIdx.x;
   for (; i + (U - 1) * stride < nvec; i += U * stride) {
     V a[U], b[U];
 #pragma unroll
@@ -320,8 +285,8 @@ __global__ __launch_bounds__(kThreads) void reduce_vec_kernel(
     const int64_t t = nvec * VN + threadIdx.x;
     if (t < n) s += (double)red(x[t], Red::two ? y[t] : T(0));
   }
-  double part = block_sum(s, lds);
-  tree_finish<T>(part, partials, tickets, result, lds);
+  const double part = block_sum<TH>(s, lds);
+  if (threadIdx.x == 0) partials[blockIdx.x] = part;
 }
 
 // Strided / batched reduction: element (b, i) at b*bs + blas_index(i, n, inc).
@@ -329,7 +294,7 @@ template <typename T, typename Red>
 __global__ __launch_bounds__(kThreads) void reduce_strided_kernel(
     int64_t n, int64_t batch, const T *__restrict__ x, int64_t incx,
     int64_t bsx, const T *__restrict__ y, int64_t incy, int64_t bsy,
-    double *partials, unsigned int *tickets, T *result, Red red) {
+    double *partials, Red red) {
   __shared__ double lds[kThreads / 64];
   const int64_t total = n * batch;
   const int64_t stride = (int64_t)gridDim.x * kThreads;
@@ -341,8 +306,8 @@ __global__ __launch_bounds__(kThreads) void reduce_strided_kernel(
     T yv = Red::two ? y[b * bsy + blas_index(i, n, incy)] : T(0);
     acc += red(xv, yv);
   }
-  double part = block_sum((double)acc, lds);
-  tree_finish<T>(part, partials, tickets, result, lds);
+  const double part = block_sum((double)acc, lds);
+  if (threadIdx.x == 0) partials[blockIdx.x] = part;
 }
 
 // ---------------------------------------------------------------------------
@@ -399,54 +364,83 @@ hcblasStatus_t run_elementwise(hcblasHandle_t h, int64_t n, int64_t batch,
   return launch_status();
 }
 
+// Waits until the reduction of call `seq` has published its result in the
+// handle's pinned block (see tree_finish).
+hcblasStatus_t wait_result(hcblasHandle_t h, unsigned long long seq) {
+  const unsigned long long *w =
+      reinterpret_cast<const unsigned long long *>(h->h_result + 1);
+  for (unsigned spin = 1;; ++spin) {
+    if (__atomic_load_n(w, __ATOMIC_ACQUIRE) == seq)
+      return HCBLAS_STATUS_SUCCESS;
+    if ((spin & 1023u) == 0) {
+      const hipError_t q = hipStreamQuery(h->stream);
+      if (q == hipSuccess)  // drained: the kernel's stores are visible now
+        return __atomic_load_n(w, __ATOMIC_ACQUIRE) == seq
+                   ? HCBLAS_STATUS_SUCCESS
+                   : HCBLAS_STATUS_EXECUTION_FAILED;
+      if (q != hipErrorNotReady) return HCBLAS_STATUS_EXECUTION_FAILED;
+    }
+    __builtin_ia32_pause();
+  }
+}
+
 template <typename T, typename Red>
 hcblasStatus_t run_reduce(hcblasHandle_t h, int64_t n, int64_t batch,
                           const T *x, int64_t incx, int64_t bsx, const T *y,
                           int64_t incy, int64_t bsy, T *result, Red red) {
   constexpr int VN = V16<T>::n;
   T *dres = reinterpret_cast<T *>(h->h_result_dev);
+  unsigned long long *done =
+      reinterpret_cast<unsigned long long *>(h->h_result_dev + 1);
+  const unsigned long long seq = ++h->result_seq;
   const int64_t flat = n * batch;
   const bool contiguous = incx == 1 && (!Red::two || incy == 1) &&
                           (batch == 1 || (bsx == n && (!Red::two || bsy == n)));
   const T *yy = Red::two ? y : x;
+  int nparts = 1;
   if (contiguous && aligned16(x) && (!Red::two || aligned16(y))) {
-    // Same bytes in flight per lane for one stream (asum) as for two (dot);
-    // FP32 asum keeps twice that in flight (SASUM 2^28: 6.05 against 5.86
-    // TB/s), which FP64 asum cannot afford (3.3 TB/s: its FP32-unrolled
-    // accumulators and loads no longer fit the registers).
+    // FP32 asum keeps 16 vectors in flight per lane (SASUM 2^28: 6.05
+    // against 5.86 TB/s with 8).
+    // FP64 asum runs 512-thread workgroups with 4 vectors in flight (DASUM
+    // 2^27 151.7 us against 165.2 us for 256 x 8 in a sweep on MI355X; the
+    // other forms measured fastest at 256 threads).
+    constexpr bool wide = !Red::two && sizeof(T) == 8;
+    constexpr int TH = wide ? 2 * kThreads : kThreads;
     constexpr int U = Red::two ? kReduceUnroll
-                               : (sizeof(T) == 4 ? 4 : 2) * kReduceUnroll;
+                      : wide   ? kReduceUnroll
+                               : 4 * kReduceUnroll;
     const int64_t nvec = flat / VN;
     const int grid = (int)std::max<int64_t>(
-        1, std::min<int64_t>(cdiv(nvec, kThreads * U), kReduceGrid));
+        1, std::min<int64_t>(cdiv(nvec, TH * U), kReduceGrid));
     const int64_t bytes = flat * (int64_t)sizeof(T) * (Red::two ? 2 : 1);
     if (bytes >= kStreamBytes)
-      reduce_vec_kernel<T, Red, true, U>
-          <<<grid, kThreads, 0, h->stream>>>(flat, x, yy, h->d_partials,
-                                             h->d_ticket, dres, red);
+      reduce_vec_kernel<T, Red, true, U, TH>
+          <<<grid, TH, 0, h->stream>>>(flat, x, yy, h->d_partials, red);
     else
-      reduce_vec_kernel<T, Red, false, U>
-          <<<grid, kThreads, 0, h->stream>>>(flat, x, yy, h->d_partials,
-                                             h->d_ticket, dres, red);
+      reduce_vec_kernel<T, Red, false, U, TH>
+          <<<grid, TH, 0, h->stream>>>(flat, x, yy, h->d_partials, red);
+    nparts = grid;
   } else if (contiguous) {
-    reduce_strided_kernel<T, Red>
-        <<<(int)std::min<int64_t>(full_grid(flat), kReduceGrid), kThreads, 0,
-           h->stream>>>(flat, 1, x, 1, 0, yy, 1, 0, h->d_partials,
-                        h->d_ticket, dres, red);
+    nparts = (int)std::min<int64_t>(full_grid(flat), kReduceGrid);
+    reduce_strided_kernel<T, Red><<<nparts, kThreads, 0, h->stream>>>(
+        flat, 1, x, 1, 0, yy, 1, 0, h->d_partials, red);
   } else {
-    reduce_strided_kernel<T, Red>
-        <<<(int)std::min<int64_t>(full_grid(flat), kReduceGrid), kThreads, 0,
-           h->stream>>>(n, batch, x, incx, bsx, yy, incy, bsy, h->d_partials,
-                        h->d_ticket, dres, red);
+    nparts = (int)std::min<int64_t>(full_grid(flat), kReduceGrid);
+    reduce_strided_kernel<T, Red><<<nparts, kThreads, 0, h->stream>>>(
+        n, batch, x, incx, bsx, yy, incy, bsy, h->d_partials, red);
   }
+  reduce_finish_kernel<T><<<1, kThreads, 0, h->stream>>>(
+      nparts, h->d_partials, dres, done, seq);
   hcblasStatus_t st = launch_status();
   if (st != HCBLAS_STATUS_SUCCESS) return st;
-  // The API returns a host scalar, so the call completes before returning
-  // (same contract as sdot_array.cpp:136-158). The kernel wrote the result
-  // into the pinned host word; the kernel's completion makes it visible.
-  if (hipStreamSynchronize(h->stream) != hipSuccess)
-    return HCBLAS_STATUS_EXECUTION_FAILED;
-  *result = *reinterpret_cast<T *>(h->h_result);
+  // The API returns a host scalar, so the call waits for its result (same
+  // contract as sdot_array.cpp:136-158): it spins on the sequence word the
+  // kernel publishes after the result, and returns as soon as it appears,
+  // without waiting for the kernel's end-of-kernel signal. If the stream
+  // drains without the word appearing, the launch failed.
+  st = wait_result(h, seq);
+  if (st != HCBLAS_STATUS_SUCCESS) return st;
+  *result = *reinterpret_cast<volatile T *>(h->h_result);
   return HCBLAS_STATUS_SUCCESS;
 }
 
