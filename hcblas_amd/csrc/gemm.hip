@@ -631,6 +631,25 @@ hcblasStatus_t gemm_common(hcblasHandle_t h, hcblasDatatype_t dt,
                   force_mode ? force_mode : h->math_mode);
 }
 
+// C = alpha * op(P) * op(Q) (beta = 0) where P is FP32 and Q BF16
+// (p_f32) or the reverse; output FP32. Used by the BF16 adjoints.
+hcblasStatus_t mixed_gemm(hcblasHandle_t h, hcblasOrder_t order,
+                          hcblasOperation_t tp, hcblasOperation_t tq, int M,
+                          int N, int K, float alpha, const void *P,
+                          int64_t ldp, int64_t sP, bool p_f32, const void *Q,
+                          int64_t ldq, int64_t sQ, float *C, int64_t ldc,
+                          int64_t sC, int64_t batch) {
+  Canon c{};
+  hcblasStatus_t st = canonicalize(order, tp, tq, M, N, K, P, ldp, 0, sP,
+                                   nullptr, Q, ldq, 0, sQ, nullptr, ldc, c);
+  if (st != HCBLAS_STATUS_SUCCESS) return st;
+  if (C == nullptr || P == nullptr || Q == nullptr)
+    return HCBLAS_STATUS_INVALID_VALUE;
+  // the row-major form swaps the operands
+  const bool a_f32 = order == HCBLAS_COL_MAJOR ? p_f32 : !p_f32;
+  return gemm_mixed_bf16(h, c, a_f32, alpha, C, ldc, sC, batch);
+}
+
 }  // namespace
 
 extern "C" {
@@ -900,6 +919,37 @@ hcblasStatus_t hcblas_gemm_strided_batched_backward(
   const double zd = 0.0;
   const void *zero = wt == HCBLAS_R_64F ? (const void *)&zd : (const void *)&zf;
   const void *Aw = A, *Bw = B;
+  if (dtype == HCBLAS_R_16BF && M > 0 && N > 0 && K > 0) {
+    // BF16 operands are read as they are by the mixed kernel (FP32 dC split
+    // into three BF16 planes on the fly); same products as below.
+    if ((dA && B == nullptr) || (dB && A == nullptr))
+      return HCBLAS_STATUS_INVALID_VALUE;
+    const float al = *static_cast<const float *>(alpha);
+    hcblasStatus_t st = HCBLAS_STATUS_SUCCESS;
+    if (dA) {
+      st = !ta ? mixed_gemm(handle, order, OP_N, tb ? OP_N : OP_T, M, K, N, al,
+                            dC, lddc, strideDC, true, B, ldb, strideB,
+                            static_cast<float *>(dA), ldda, strideDA,
+                            batchCount)
+               : mixed_gemm(handle, order, tb ? OP_T : OP_N, OP_T, K, M, N, al,
+                            B, ldb, strideB, false, dC, lddc, strideDC,
+                            static_cast<float *>(dA), ldda, strideDA,
+                            batchCount);
+      if (st != HCBLAS_STATUS_SUCCESS) return st;
+    }
+    if (dB) {
+      st = !tb ? mixed_gemm(handle, order, ta ? OP_N : OP_T, OP_N, K, N, M, al,
+                            A, lda, strideA, false, dC, lddc, strideDC,
+                            static_cast<float *>(dB), lddb, strideDB,
+                            batchCount)
+               : mixed_gemm(handle, order, OP_T, ta ? OP_T : OP_N, N, K, M, al,
+                            dC, lddc, strideDC, true, A, lda, strideA,
+                            static_cast<float *>(dB), lddb, strideDB,
+                            batchCount);
+      if (st != HCBLAS_STATUS_SUCCESS) return st;
+    }
+    dA = dB = nullptr;  // done; dC_in below
+  }
   if (dtype == HCBLAS_R_16BF && (dA || dB)) {
     // A is rows_a x cols_a in storage, B rows_b x cols_b.
     const int ra = ta ? K : M, ca = ta ? M : K;

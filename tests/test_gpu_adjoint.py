@@ -80,44 +80,75 @@ def _run_backward(hc, handle, dtype, order, ta, tb, M, N, K, alpha, beta, A,
     return to_host(gA), to_host(gB), to_host(gC)
 
 
-@pytest.mark.parametrize("p", ["s", "d"])
+def _bf16_bits(x):
+    """Round-to-nearest-even BF16 bit patterns (uint16) of float32 x."""
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+
+
+def _bf16_value(b):
+    return (b.astype(np.uint32) << 16).view(np.float32)
+
+
+@pytest.mark.parametrize("p,shape", [("s", "odd"), ("d", "odd"),
+                                     ("bf", "odd"), ("bf", "aligned")])
 @pytest.mark.parametrize("order", [COL, ROW])
 @pytest.mark.parametrize("ta", [N_, T_])
 @pytest.mark.parametrize("tb", [N_, T_])
-def test_gemm_adjoints_all_cases(hc, handle, orc, p, order, ta, tb):
-    M, N, K, batch = 37, 29, 45, 3
+def test_gemm_adjoints_all_cases(hc, handle, orc, p, shape, order, ta, tb):
+    """dA, dB, dC_in of the strided batched GEMM in every order x trans case
+    with padded leading dimensions and batch strides; "bf": BF16 A and B
+    (the mixed kernel: FP32 dC split into BF16 planes), FP32 adjoints.
+    Componentwise bound 4 sqrt(n) u |alpha| (|dC| |op(B)^T|) + 2 u |ref|
+    (n the inner length, u the unit roundoff of the FP32 / FP64 result)."""
+    if shape == "odd":
+        M, N, K, batch, pad = 37, 29, 45, 3, (3, 1, 2)
+    else:
+        M, N, K, batch, pad = 96, 64, 160, 2, (8, 8, 8)
     alpha, beta = 0.75, -1.25
-    dt = np.float32 if p == "s" else np.float64
-    gen = orc.uniform_f32 if p == "s" else orc.uniform_f64
+    dt = np.float64 if p == "d" else np.float32
+    gen = orc.uniform_f64 if p == "d" else orc.uniform_f32
     ra, ca = (K, M) if ta else (M, K)
     rb, cb = (N, K) if tb else (K, N)
-    lda = (ra if order == COL else ca) + 3
-    ldb = (rb if order == COL else cb) + 1
-    ldw = (M if order == COL else N) + 2
-    sA = _storage(order, ra, ca, lda, 1, 0) + 5
-    sB = _storage(order, rb, cb, ldb, 1, 0) + 7
-    sW = _storage(order, M, N, ldw, 1, 0) + 1
+    lda = (ra if order == COL else ca) + pad[0]
+    ldb = (rb if order == COL else cb) + pad[1]
+    ldw = (M if order == COL else N) + pad[2]
+    al = 8 if shape == "aligned" else 1
+    sA = _storage(order, ra, ca, lda, 1, 0) + (5 if al == 1 else 8)
+    sB = _storage(order, rb, cb, ldb, 1, 0) + (7 if al == 1 else 16)
+    sW = _storage(order, M, N, ldw, 1, 0) + (1 if al == 1 else 8)
     A = gen(_storage(order, ra, ca, lda, batch, sA), 1)
     B = gen(_storage(order, rb, cb, ldb, batch, sB), 2)
     W = gen(_storage(order, M, N, ldw, batch, sW), 3)
-    ldda, lddb, ldci = lda + 1, ldb + 2, ldw + 1
-    sdA = _storage(order, ra, ca, ldda, 1, 0) + 2
+    dtype = {"s": 0, "d": 1, "bf": 5}[p]
+    if p == "bf":
+        Ab, Bb = _bf16_bits(A), _bf16_bits(B)
+        A, B = _bf16_value(Ab), _bf16_value(Bb)  # the values the GPU sees
+        dev_A, dev_B = Ab, Bb
+    else:
+        dev_A, dev_B = A, B
+    ldda, lddb, ldci = lda + al, ldb + 2 * al, ldw + al
+    sdA = _storage(order, ra, ca, ldda, 1, 0) + 2 * al
     sdB = _storage(order, rb, cb, lddb, 1, 0)
-    sCi = _storage(order, M, N, ldci, 1, 0) + 3
-    dtype = 0 if p == "s" else 1
+    sCi = _storage(order, M, N, ldci, 1, 0) + 3 * al
     gA, gB, gC = _run_backward(hc, handle, dtype, order, ta, tb, M, N, K,
-                               alpha, beta, A, lda, sA, B, ldb, sB, W, ldw,
-                               sW, batch, ldda, sdA, lddb, sdB, ldci, sCi, dt)
+                               alpha, beta, dev_A, lda, sA, dev_B, ldb, sB, W,
+                               ldw, sW, batch, ldda, sdA, lddb, sdB, ldci, sCi,
+                               dt)
     refs = _ref_adjoints(orc, order, ta, tb, M, N, K, alpha, beta, A, lda, sA,
                          B, ldb, sB, W, ldw, sW, batch)
-    eps = np.finfo(dt).eps
-    for b, (rA, rB, rC) in enumerate(refs):
+    absr = _ref_adjoints(orc, order, ta, tb, M, N, K, abs(alpha), 0.0,
+                         np.abs(A), lda, sA, np.abs(B), ldb, sB, np.abs(W),
+                         ldw, sW, batch)
+    u = np.finfo(dt).eps / 2
+    for b, ((rA, rB, rC), (pA, pB, _)) in enumerate(zip(refs, absr)):
         gotA = _mat(orc, gA[b * sdA:], order, N_, ra, ca, ldda)
         gotB = _mat(orc, gB[b * sdB:], order, N_, rb, cb, lddb)
         gotC = _mat(orc, gC[b * sCi:], order, N_, M, N, ldci)
-        # |alpha| * (inner length + 2) * eps * max|entries products|
-        assert np.max(np.abs(gotA - rA)) <= 2 * alpha * (N + 2) * eps * N, b
-        assert np.max(np.abs(gotB - rB)) <= 2 * alpha * (M + 2) * eps * M, b
+        assert np.all(np.abs(gotA - rA) <=
+                      4 * np.sqrt(N) * u * pA + 2 * u * np.abs(rA)), b
+        assert np.all(np.abs(gotB - rB) <=
+                      4 * np.sqrt(M) * u * pB + 2 * u * np.abs(rB)), b
         # dC_in = beta * W is one rounding of the product: exact match
         Wm = _mat(orc, W[b * sW:], order, N_, M, N, ldw)
         assert bitwise_equal(gotC.astype(dt), (dt(beta) * Wm).astype(dt)), b
@@ -198,14 +229,19 @@ def test_bf16_config_adjoints(hc, handle, orc, torch_cuda):
     Wh = W.cpu().numpy().astype(np.float64).reshape(batch, M, N)
     gA = dA.cpu().numpy().reshape(batch, M, K)
     gB = dB.cpu().numpy().reshape(batch, K, N)
-    eps = 2.0 ** -23
+    # componentwise FP32 bound: 4 sqrt(n) u |alpha| (|W| |B^T|) + 2 u |ref|
+    u = 2.0 ** -24
     for b in rng.integers(0, batch, 8):
         i = rng.integers(0, M)
         ref = alpha * Wh[b, i, :] @ Bh[b].T          # row i of dA[b]
-        assert np.max(np.abs(gA[b, i] - ref)) <= 2 * (N + 2) * eps * N
+        absp = abs(alpha) * np.abs(Wh[b, i, :]) @ np.abs(Bh[b]).T
+        bound = 4 * np.sqrt(N) * u * absp + 2 * u * np.abs(ref)
+        assert np.all(np.abs(gA[b, i] - ref) <= bound)
         j = rng.integers(0, N)
         ref = alpha * Ah[b].T @ Wh[b, :, j]          # column j of dB[b]
-        assert np.max(np.abs(gB[b, :, j] - ref)) <= 2 * (M + 2) * eps * M
+        absp = abs(alpha) * np.abs(Ah[b]).T @ np.abs(Wh[b, :, j])
+        bound = 4 * np.sqrt(M) * u * absp + 2 * u * np.abs(ref)
+        assert np.all(np.abs(gB[b, :, j] - ref) <= bound)
     assert bitwise_equal(dCi.cpu().numpy(), (np.float32(beta) * W.cpu().numpy()))
 
 
