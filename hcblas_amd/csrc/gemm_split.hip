@@ -680,7 +680,7 @@ struct ResidueWeights {
 // the distance 1/(2m) of s/m from a half-integer), and s - q*m is exact.
 template <int NMOD, bool WIDE, int T = 0>
 __device__ __forceinline__ void write_residues16(const ResidueIn<WIDE> &in,
-                                                 int8_t *row, int64_t Kp) {
+                                                 int8_t *row, int64_t pstride) {
   if constexpr (T < NMOD) {
     constexpr int M = modulus<T>();
     int q[16];
@@ -713,8 +713,8 @@ __device__ __forceinline__ void write_residues16(const ResidueIn<WIDE> &in,
     w.y = pack_bytes(q[4], q[5], q[6], q[7]);
     w.z = pack_bytes(q[8], q[9], q[10], q[11]);
     w.w = pack_bytes(q[12], q[13], q[14], q[15]);
-    *reinterpret_cast<vec4u *>(row + (int64_t)T * Kp) = w;
-    write_residues16<NMOD, WIDE, T + 1>(in, row, Kp);
+    *reinterpret_cast<vec4u *>(row + (int64_t)T * pstride) = w;
+    write_residues16<NMOD, WIDE, T + 1>(in, row, pstride);
   }
 }
 
@@ -723,14 +723,22 @@ __device__ __forceinline__ void write_residues16(const ResidueIn<WIDE> &in,
 // LDS, reading along the source's contiguous dimension; each thread then
 // converts 16 consecutive k of one row and writes one 16-byte vector per
 // plane, so eight threads write a whole 128-byte line of each plane row
-// (a 64-wide tile wrote half lines, which the memory merges less well). FP32 input: bits - e_r keeps |v| <= 2^30, and the scaling and
-// rounding are exact in float (values below 2^-126 round to 0 either way);
-// FP64 input: |v| <= 2^62, split exactly into 32-bit halves in double.
+// (a 64-wide tile wrote half lines, which the memory merges less well).
+// FP32 input: bits - e_r keeps |v| <= 2^30, and the scaling and rounding are
+// exact in float (values below 2^-126 round to 0 either way); FP64 input:
+// |v| <= 2^62, split exactly into 32-bit halves in double.
+//
+// With fstride != 0 the planes are written in MFMA fragment order instead:
+// plane t starts at P + t * fstride (fstride = Rpad * Kp, Rpad a multiple
+// of 256), and the 32 x 32 block (r / 32, k / 32) is one 1 KiB run holding
+// the 16 bytes of (r, 16-k group g) at lane (r % 32) + 32 * (g % 2) -- the
+// operand layout of v_mfma_i32_32x32x32_i8, so a wave loads a fragment with
+// one coalesced 16-byte load per lane. Rows from R to Rpad are not written.
 template <typename T, int NMOD>
 __global__ __launch_bounds__(256) void crt_residue_kernel(
     int R, int K, const T *__restrict__ X, int64_t ld, bool kcont, bool vec,
     const int *__restrict__ exps, int bits, int8_t *__restrict__ P,
-    int64_t Kp) {
+    int64_t Kp, int64_t fstride) {
   constexpr bool WIDE = sizeof(T) == 8;
   __shared__ TileRK<T, 32, 128> st;
   const int r0 = blockIdx.y * 32, k0 = blockIdx.x * 128;
@@ -758,14 +766,23 @@ __global__ __launch_bounds__(256) void crt_residue_kernel(
       in.lo[j] = (uint32_t)(int)v + (1u << 30);
     }
   }
-  write_residues16<NMOD, WIDE>(in, P + (int64_t)r * NMOD * Kp + k0 + kq, Kp);
+  if (fstride == 0) {
+    write_residues16<NMOD, WIDE>(in, P + (int64_t)r * NMOD * Kp + k0 + kq, Kp);
+  } else {
+    const int k = k0 + kq;
+    const int64_t blk = (int64_t)(r >> 5) * (Kp >> 5) + (k >> 5);
+    const int lane = (r & 31) + 32 * ((k >> 4) & 1);
+    write_residues16<NMOD, WIDE>(in, P + blk * 1024 + lane * 16, fstride);
+  }
 }
 
 struct CrtParams {
   int M, N;
   int64_t Kp;
   int nmod;
-  const int8_t *Ap, *Bp;  // residue planes (rows of nmod * Kp)
+  const int8_t *Ap, *Bp;  // residue planes: A rows of nmod * Kp, B in
+                          // fragment order (plane t at Bp + t * bstride)
+  int64_t bstride;
   uint8_t *R;             // R[t][n][m]: (A'B')(m, n) mod m_t in [0, m_t)
   int tiles_m, tiles_n;
 };
@@ -835,25 +852,181 @@ __device__ __forceinline__ void crt_store_tile(const AccArr &acc, int mod,
   }
 }
 
-// One INT8 GEMM per modulus (blockIdx.z = t) over the residue planes, rows
-// of the tile along m (A planes), columns along n (B planes).
-template <int BM, int BN, int BK, int WM, int WN, int MI>
-__global__ __launch_bounds__(64 * WM * WN) void crt_gemm_kernel(CrtParams p) {
-  constexpr int LDS_BYTES = tile_lds_bytes<Elem::I8, BM, BN, BK>();
-  __shared__ __attribute__((aligned(16))) char lds[LDS_BYTES];
-  using WT = WaveTile<Elem::I8, BM, BN, WM, WN, MI>;
+// Main loop of the residue GEMM: a 256 x 256 tile of one modulus' product
+// over INT8 planes whose length Kp is a multiple of 128 (zero padded), 4
+// waves of 128 x 128 (4 x 4 blocks of v_mfma_i32_32x32x32_i8), one wave per
+// SIMD. A comes from k-contiguous rows (row m at Ap + m * lda), B from
+// fragment-ordered planes (crt_residue_kernel with fstride != 0).
+//
+// On gfx950 the LDS moves about 100 bytes per clock per CU for
+// ds_write_b128 and not much more for reads, so a 256 x 256 x 128 step
+// staging both operands (64 KiB written, 128 KiB read by 4 waves) keeps the
+// LDS about as busy as the 2048 MFMA cycles per SIMD of the step. Here only
+// A goes through LDS (32 KiB written, 64 KiB read); each wave loads its B
+// fragments itself, one 1 KiB coalesced load per fragment, from planes
+// laid out in MFMA operand order. Latency is hidden inside the wave:
+//   - the A fragments of chunk c + 1 are read while the MFMAs of chunk c
+//     run (two A fragment sets);
+//   - the B fragments of chunk c of step kt + 1 are loaded right after the
+//     MFMAs of chunk c of step kt, a whole step ahead of their use;
+//   - the staged A tile of step kt + 1 is written to the other LDS buffer
+//     during chunk 0 of step kt and the registers are refilled with step
+//     kt + 2 during chunk 1;
+//   - the one barrier of a step comes before the MFMAs of its last chunk,
+//     followed by the A reads of the next step's first chunk.
+// Rows past M load row m0 (inside the planes); B rows past N are inside the
+// padded planes. Either only feeds outputs the epilogue never stores.
+constexpr int kCrtBM = 256, kCrtBK = 128, kCrtRowB = kCrtBK + 16;
+constexpr int kCrtBuf = kCrtBM * kCrtRowB;  // the A tile of one step
+constexpr int kCrtLds = 2 * kCrtBuf;
+
+using CrtAcc = intx16[4][4];
+
+__device__ __forceinline__ void crt_mainloop(const int8_t *__restrict__ Ap,
+                                             int64_t lda,
+                                             const int8_t *__restrict__ Bf,
+                                             int M, int Kp, int m0, int n0,
+                                             char *lds, CrtAcc &acc) {
+  constexpr int NTH = 256, PT = kCrtBM * (kCrtBK / 16) / NTH;  // 8
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int lrow = lane & 31, lgrp = lane >> 5;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = intx16{};
+  // A loader: vector v = tid + NTH * i is row v / 8, bytes (v % 8) * 16.
+  const int lr = tid >> 3, lk = (tid & 7) * 16;
+  const int8_t *ga[PT];
+#pragma unroll
+  for (int i = 0; i < PT; ++i) {
+    const int ra = m0 + lr + 32 * i;
+    ga[i] = Ap + (int64_t)(ra < M ? ra : m0) * lda + lk;
+  }
+  vec4u sa[PT];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int i = 0; i < PT; ++i)
+      sa[i] = *reinterpret_cast<const vec4u *>(ga[i] + k0);
+  };
+  const int soff = lr * kCrtRowB + lk;
+  auto store_a = [&](char *b) {
+#pragma unroll
+    for (int i = 0; i < PT; ++i)
+      *reinterpret_cast<vec4u *>(b + soff + 32 * i * kCrtRowB) = sa[i];
+  };
+  auto buf = [&](int kt) { return lds + (kt & 1) * kCrtBuf; };
+  const int aoff = (wm * 128 + lrow) * kCrtRowB + lgrp * 16;
+  vec4u fa[2][4], fb[4][4];
+  auto frags = [&](vec4u(&a)[4], const char *cur, int c) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      a[i] = *reinterpret_cast<const vec4u *>(cur + aoff + i * 32 * kCrtRowB +
+                                              c * 32);
+  };
+  // B fragment (block j, chunk q) at bbase + j * jstride + q * 1024.
+  const int64_t jstride = (int64_t)Kp * 32;
+  const int8_t *bbase =
+      Bf + (int64_t)((n0 + wn * 128) >> 5) * jstride + lane * 16;
+  auto bload = [&](int kt, int c) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      fb[c][j] = *reinterpret_cast<const vec4u *>(
+          bbase + j * jstride + (int64_t)(4 * kt + c) * 1024);
+  };
+  auto mfmas = [&](const vec4u(&a)[4], int c) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(
+            bitcast<intx4>(a[i]), bitcast<intx4>(fb[c][j]), acc[i][j], 0, 0,
+            0);
+  };
+  const int nk = Kp / kCrtBK;
+  if (nk == 0) return;
+  gload(0);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) bload(0, c);
+  store_a(buf(0));
+  if (nk > 1) gload(kCrtBK);
+  __syncthreads();
+  frags(fa[0], buf(0), 0);
+  // Issue order inside a chunk: one MFMA, then one of the chunk's memory
+  // instructions, so the MFMA pipe stays busy while they are sent.
+  // The closing sched_barrier keeps each chunk's instructions in its own
+  // region, so loads are issued in the order they are needed.
+  auto interleave = [](int reads, int writes, int vmem) {
+    for (int i = 0; i < 16; ++i) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+      if (i < reads) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+      if (i >= 16 - writes)
+        __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
+      if (i >= 16 - vmem) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  };
+  int kt = 0;
+  // Steady state: no conditions inside a step, so each chunk is one
+  // scheduling region.
+  for (; kt + 2 < nk; ++kt) {
+    const char *cur = buf(kt);
+    char *nxt = buf(kt + 1);
+    frags(fa[1], cur, 1);
+    store_a(nxt);
+    mfmas(fa[0], 0);
+    bload(kt + 1, 0);
+    interleave(4, 8, 4);
+    frags(fa[0], cur, 2);
+    gload((kt + 2) * kCrtBK);
+    mfmas(fa[1], 1);
+    bload(kt + 1, 1);
+    interleave(4, 0, 12);
+    frags(fa[1], cur, 3);
+    mfmas(fa[0], 2);
+    bload(kt + 1, 2);
+    interleave(4, 0, 4);
+    __syncthreads();
+    frags(fa[0], nxt, 0);
+    mfmas(fa[1], 3);
+    bload(kt + 1, 3);
+    interleave(4, 0, 4);
+  }
+  for (; kt < nk; ++kt) {
+    const char *cur = buf(kt);
+    char *nxt = buf(kt + 1);
+    const bool more = kt + 1 < nk;
+    frags(fa[1], cur, 1);
+    if (more) store_a(nxt);
+    mfmas(fa[0], 0);
+    if (more) bload(kt + 1, 0);
+    frags(fa[0], cur, 2);
+    mfmas(fa[1], 1);
+    if (more) bload(kt + 1, 1);
+    frags(fa[1], cur, 3);
+    mfmas(fa[0], 2);
+    if (more) bload(kt + 1, 2);
+    __syncthreads();
+    if (more) frags(fa[0], nxt, 0);
+    mfmas(fa[1], 3);
+    if (more) bload(kt + 1, 3);
+  }
+}
+
+// One INT8 GEMM per modulus (blockIdx.z = t) with the 4-wave main loop.
+__global__ __launch_bounds__(256) void crt_gemm4_kernel(CrtParams p) {
+  __shared__ __attribute__((aligned(16))) char lds[kCrtLds];
   const int t = blockIdx.z;
   const int mod = d_moduli[t];
   int tm, tn;
   tile_coords(blockIdx.x, p.tiles_m, p.tiles_n, tm, tn);
-  const int64_t ld = (int64_t)p.nmod * p.Kp;
-  typename WT::AccV acc[WT::FM][WT::FN];
-  gemm_mainloop<Elem::I8, true, true, BM, BN, BK, WM, WN, true, true, MI,
-                true>(p.Ap + t * p.Kp, ld, p.Bp + t * p.Kp, ld, p.M, p.N,
-                      (int)p.Kp, tm * BM, tn * BN, lds, acc);
-  crt_store_tile<BM, BN, WM, WN, LDS_BYTES>(
-      acc, mod, p.R + (int64_t)t * p.M * p.N, p.M, p.N, tm * BM, tn * BN,
-      lds);
+  CrtAcc acc;
+  crt_mainloop(p.Ap + t * p.Kp, (int64_t)p.nmod * p.Kp, p.Bp + t * p.bstride,
+               p.M, (int)p.Kp, tm * kCrtBM, tn * kCrtBM, lds, acc);
+  crt_store_tile<kCrtBM, kCrtBM, 2, 2, kCrtLds>(
+      acc, mod, p.R + (int64_t)t * p.M * p.N, p.M, p.N, tm * kCrtBM,
+      tn * kCrtBM, lds);
 }
 
 // Reconstruction by the explicit CRT. With P = prod m_t, M_t = P / m_t and
@@ -1058,11 +1231,11 @@ __global__ __launch_bounds__(256) void crt_combine_kernel(
 template <int NMOD, typename T>
 void launch_crt_residues(hipStream_t s, int R, int K, const T *X, int64_t ld,
                          bool kcont, const int *exps, int bits, int8_t *P,
-                         int64_t Kp) {
+                         int64_t Kp, int64_t fstride) {
   // Kp is a multiple of 128 (the INT8 GEMM's K step).
   dim3 g((unsigned)(Kp / 128), (unsigned)cdiv(R, 32));
   crt_residue_kernel<T, NMOD><<<g, 256, 0, s>>>(
-      R, K, X, ld, kcont, vec_aligned(X, ld), exps, bits, P, Kp);
+      R, K, X, ld, kcont, vec_aligned(X, ld), exps, bits, P, Kp, fstride);
 }
 template <int NMOD, typename OutT>
 void launch_crt_combine(hipStream_t s, int M, int N, const uint8_t *R,
@@ -1280,12 +1453,7 @@ hcblasStatus_t gemm_ozaki_crt(hcblasHandle_t h, const Canon &c, double alpha,
   if (alpha == 0.0 || c.K == 0)
     return gemm_real<Elem::F64, double>(h, c, alpha, beta, C, ldc, cOff,
                                         strideC, Cptrs, batch);
-  // 256x256 tiles, 8 waves of 128x64 on 32x32x32 INT8 MFMA blocks (8.4 ms
-  // for 8192^3 x 16 moduli, 42% of INT8 peak). Reading the next k chunk's
-  // fragments ahead of the current MFMAs (a second fragment set, 237 VGPRs)
-  // measured no faster (8.5 ms): with two waves per SIMD the LDS latency is
-  // already covered.
-  constexpr int BM = 256, BN = 256, BK = 128, WM = 2, WN = 4, MI = 32;
+  constexpr int BK = kCrtBK;
   // INT32 exactness of one modulus product: kc * 128 * 128 < 2^31.
   constexpr int64_t kmax = 65536;
   const int64_t kchunk = std::min<int64_t>(cdiv(c.K, BK) * BK, kmax);
@@ -1296,7 +1464,9 @@ hcblasStatus_t gemm_ozaki_crt(hcblasHandle_t h, const Canon &c, double alpha,
                        : std::max(crt_auto_moduli((int)kchunk),
                                   crt_auto_moduli((int)(c.K % kchunk ? c.K % kchunk : kchunk)));
   const size_t a_bytes = (size_t)nmax * c.M * Kp;
-  const size_t b_bytes = (size_t)nmax * c.N * Kp;
+  // B planes in fragment order over N rounded up to the 256-column tile.
+  const int64_t Npad = cdiv(c.N, kCrtBM) * kCrtBM;
+  const size_t b_bytes = (size_t)nmax * Npad * Kp;
   const size_t r_bytes = (size_t)nmax * c.M * c.N;
   const size_t e_bytes = sizeof(int) * ((size_t)c.M + c.N);
   auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
@@ -1330,9 +1500,11 @@ hcblasStatus_t gemm_ozaki_crt(hcblasHandle_t h, const Canon &c, double alpha,
       launch_row_exp(h->stream, c.M, kc, A, c.lda, c.a_k, ea);
       launch_row_exp(h->stream, c.N, kc, B, c.ldb, c.b_k, eb);
       if (!with_nmod<ResidueLaunch>(nmod, h->stream, c.M, kc, A, c.lda,
-                                    c.a_k, (const int *)ea, bits, Ap, Kp) ||
+                                    c.a_k, (const int *)ea, bits, Ap, Kp,
+                                    (int64_t)0) ||
           !with_nmod<ResidueLaunch>(nmod, h->stream, c.N, kc, B, c.ldb,
-                                    c.b_k, (const int *)eb, bits, Bp, Kp))
+                                    c.b_k, (const int *)eb, bits, Bp, Kp,
+                                    Npad * Kp))
         return HCBLAS_STATUS_INVALID_VALUE;
       st = launch_status();
       if (st != HCBLAS_STATUS_SUCCESS) return st;
@@ -1343,12 +1515,13 @@ hcblasStatus_t gemm_ozaki_crt(hcblasHandle_t h, const Canon &c, double alpha,
       p.nmod = nmod;
       p.Ap = Ap;
       p.Bp = Bp;
+      p.bstride = Npad * Kp;
       p.R = R;
-      p.tiles_m = (int)cdiv(c.M, BM);
-      p.tiles_n = (int)cdiv(c.N, BN);
-      crt_gemm_kernel<BM, BN, BK, WM, WN, MI>
-          <<<dim3((unsigned)(p.tiles_m * p.tiles_n), 1, (unsigned)nmod),
-             64 * WM * WN, 0, h->stream>>>(p);
+      p.tiles_m = (int)cdiv(c.M, kCrtBM);
+      p.tiles_n = (int)cdiv(c.N, kCrtBM);
+      crt_gemm4_kernel<<<dim3((unsigned)(p.tiles_m * p.tiles_n), 1,
+                              (unsigned)nmod),
+                         256, 0, h->stream>>>(p);
       st = launch_status();
       if (st != HCBLAS_STATUS_SUCCESS) return st;
       with_nmod<CombineLaunch>(nmod, h->stream, c.M, c.N, (const uint8_t *)R,
@@ -1390,7 +1563,7 @@ hcblasStatus_t gemm_ozaki_crt_f32(hcblasHandle_t h, const Canon &c,
   if (alpha == 0.f || c.K == 0)
     return gemm_real<Elem::F32, float>(h, c, alpha, beta, C, ldc, cOff,
                                        strideC, Cptrs, batch);
-  constexpr int BM = 256, BN = 256, BK = 128, WM = 2, WN = 4, MI = 32;
+  constexpr int BK = kCrtBK;
   constexpr int64_t kmax = 65536;  // INT32 exactness: kc * 128 * 128 < 2^31
   const int64_t kchunk = std::min<int64_t>(cdiv(c.K, BK) * BK, kmax);
   const int64_t Kp = kchunk;
@@ -1398,7 +1571,8 @@ hcblasStatus_t gemm_ozaki_crt_f32(hcblasHandle_t h, const Canon &c,
   const int bb = crt_beta_f32(nmod);
   const int pa = (int)cdiv(kchunk, 64), pb = pa;  // 64-wide k blocks
   const size_t a_bytes = (size_t)nmod * c.M * Kp;
-  const size_t b_bytes = (size_t)nmod * c.N * Kp;
+  const int64_t Npad = cdiv(c.N, kCrtBM) * kCrtBM;  // B in fragment order
+  const size_t b_bytes = (size_t)nmod * Npad * Kp;
   const size_t r_bytes = (size_t)nmod * c.M * c.N;
   const size_t e_bytes = sizeof(int) * ((size_t)c.M + c.N);
   const size_t p_bytes = sizeof(double) * ((size_t)pa * c.M + (size_t)pb * c.N);
@@ -1440,9 +1614,11 @@ hcblasStatus_t gemm_ozaki_crt_f32(hcblasHandle_t h, const Canon &c,
       norms(c.M, kc, A, c.lda, c.a_k, parta, ea);
       norms(c.N, kc, B, c.ldb, c.b_k, partb, eb);
       if (!with_nmod<ResidueLaunch>(nmod, h->stream, c.M, kc, A, c.lda,
-                                    c.a_k, (const int *)ea, 0, Ap, Kp) ||
+                                    c.a_k, (const int *)ea, 0, Ap, Kp,
+                                    (int64_t)0) ||
           !with_nmod<ResidueLaunch>(nmod, h->stream, c.N, kc, B, c.ldb,
-                                    c.b_k, (const int *)eb, 0, Bp, Kp))
+                                    c.b_k, (const int *)eb, 0, Bp, Kp,
+                                    Npad * Kp))
         return HCBLAS_STATUS_INVALID_VALUE;
       st = launch_status();
       if (st != HCBLAS_STATUS_SUCCESS) return st;
@@ -1453,12 +1629,13 @@ hcblasStatus_t gemm_ozaki_crt_f32(hcblasHandle_t h, const Canon &c,
       p.nmod = nmod;
       p.Ap = Ap;
       p.Bp = Bp;
+      p.bstride = Npad * Kp;
       p.R = R;
-      p.tiles_m = (int)cdiv(c.M, BM);
-      p.tiles_n = (int)cdiv(c.N, BN);
-      crt_gemm_kernel<BM, BN, BK, WM, WN, MI>
-          <<<dim3((unsigned)(p.tiles_m * p.tiles_n), 1, (unsigned)nmod),
-             64 * WM * WN, 0, h->stream>>>(p);
+      p.tiles_m = (int)cdiv(c.M, kCrtBM);
+      p.tiles_n = (int)cdiv(c.N, kCrtBM);
+      crt_gemm4_kernel<<<dim3((unsigned)(p.tiles_m * p.tiles_n), 1,
+                              (unsigned)nmod),
+                         256, 0, h->stream>>>(p);
       st = launch_status();
       if (st != HCBLAS_STATUS_SUCCESS) return st;
       with_nmod<CombineLaunch>(nmod, h->stream, c.M, c.N, (const uint8_t *)R,
