@@ -564,6 +564,10 @@ def adjoint_section(torch, reps=5):
         return torch_ext.gemm_batched(A, B)
 
     def fwd_bwd():
+        # gradients set to None as a training step's zero_grad does, so the
+        # backward writes them instead of adding into the previous ones
+        A.grad = None
+        B.grad = None
         C = torch_ext.gemm_batched(A, B)
         C.backward(W)
 

@@ -485,15 +485,6 @@ template hcblasStatus_t gemm_real<Elem::F64, double>(hcblasHandle_t,
 namespace hcb {
 namespace {
 
-// dst[i] = float(src[i]) for bfloat16 bit patterns (exact).
-__global__ __launch_bounds__(256) void bf16_to_f32_kernel(int64_t n,
-                                                          const uint16_t *src,
-                                                          float *dst) {
-  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * 256)
-    dst[i] = __uint_as_float((uint32_t)src[i] << 16);
-}
-
 // dst = beta * src over a batch of rows x cols matrices stored with
 // (element (i, j) at i * rs + j * cs); beta == 0 writes zeros.
 template <typename T>
@@ -516,16 +507,6 @@ __global__ __launch_bounds__(256) void scaled_copy_kernel(
     }
     d[i * dst_rs + j * dst_cs] = beta == T(0) ? T(0) : beta * s[i * src_rs + j * src_cs];
   }
-}
-
-// Elements spanned by a strided batch of rows x cols matrices.
-int64_t batch_span(hcblasOrder_t order, int rows, int cols, int64_t ld,
-                   int64_t stride, int batch) {
-  if (rows == 0 || cols == 0 || batch == 0) return 0;
-  const int64_t foot = order == HCBLAS_COL_MAJOR
-                           ? ld * (cols - 1) + rows
-                           : ld * (rows - 1) + cols;
-  return (int64_t)(batch - 1) * stride + foot;
 }
 
 }  // namespace
@@ -637,7 +618,7 @@ hcblasStatus_t mixed_gemm(hcblasHandle_t h, hcblasOrder_t order,
                           hcblasOperation_t tp, hcblasOperation_t tq, int M,
                           int N, int K, float alpha, const void *P,
                           int64_t ldp, int64_t sP, bool p_f32, const void *Q,
-                          int64_t ldq, int64_t sQ, float *C, int64_t ldc,
+                          int64_t ldq, int64_t sQ, void *C, int64_t ldc,
                           int64_t sC, int64_t batch) {
   Canon c{};
   hcblasStatus_t st = canonicalize(order, tp, tq, M, N, K, P, ldp, 0, sP,
@@ -647,7 +628,7 @@ hcblasStatus_t mixed_gemm(hcblasHandle_t h, hcblasOrder_t order,
     return HCBLAS_STATUS_INVALID_VALUE;
   // the row-major form swaps the operands
   const bool a_f32 = order == HCBLAS_COL_MAJOR ? p_f32 : !p_f32;
-  return gemm_mixed_bf16(h, c, a_f32, alpha, C, ldc, sC, batch);
+  return gemm_mixed_bf16(h, c, a_f32, alpha, C, true, ldc, sC, batch);
 }
 
 }  // namespace
@@ -887,9 +868,10 @@ hcblasStatus_t hcblas_dgemm_ozaki_crt(
 //                                op(A) = A^T),
 //   dB = alpha * op(A)^T * dC   (alpha * dC^T * op(A) when op(B) = B^T),
 //   dC_in = beta * dC.
-// Each product runs on the MFMA GEMM kernels with the adjoint orientation;
-// BF16 operands are widened to FP32 once (exactly) and the adjoints are
-// computed and returned in FP32, the accumulation type of the forward.
+// Each product runs on the MFMA GEMM kernels with the adjoint orientation.
+// BF16: the mixed kernel reads A and B as they are and splits the FP32
+// cotangent into three BF16 planes on the fly (exact), accumulates in FP32
+// and rounds dA and dB once to BF16, the type of A and B.
 hcblasStatus_t hcblas_gemm_strided_batched_backward(
     hcblasHandle_t handle, hcblasDatatype_t dtype, hcblasOrder_t order,
     hcblasOperation_t typeA, hcblasOperation_t typeB, int M, int N, int K,
@@ -918,8 +900,7 @@ hcblasStatus_t hcblas_gemm_strided_batched_backward(
   const float zf = 0.f;
   const double zd = 0.0;
   const void *zero = wt == HCBLAS_R_64F ? (const void *)&zd : (const void *)&zf;
-  const void *Aw = A, *Bw = B;
-  if (dtype == HCBLAS_R_16BF && M > 0 && N > 0 && K > 0) {
+  if (dtype == HCBLAS_R_16BF) {
     // BF16 operands are read as they are by the mixed kernel (FP32 dC split
     // into three BF16 planes on the fly); same products as below.
     if ((dA && B == nullptr) || (dB && A == nullptr))
@@ -929,67 +910,33 @@ hcblasStatus_t hcblas_gemm_strided_batched_backward(
     if (dA) {
       st = !ta ? mixed_gemm(handle, order, OP_N, tb ? OP_N : OP_T, M, K, N, al,
                             dC, lddc, strideDC, true, B, ldb, strideB,
-                            static_cast<float *>(dA), ldda, strideDA,
-                            batchCount)
+                            dA, ldda, strideDA, batchCount)
                : mixed_gemm(handle, order, tb ? OP_T : OP_N, OP_T, K, M, N, al,
-                            B, ldb, strideB, false, dC, lddc, strideDC,
-                            static_cast<float *>(dA), ldda, strideDA,
-                            batchCount);
+                            B, ldb, strideB, false, dC, lddc, strideDC, dA,
+                            ldda, strideDA, batchCount);
       if (st != HCBLAS_STATUS_SUCCESS) return st;
     }
     if (dB) {
       st = !tb ? mixed_gemm(handle, order, ta ? OP_N : OP_T, OP_N, K, N, M, al,
                             A, lda, strideA, false, dC, lddc, strideDC,
-                            static_cast<float *>(dB), lddb, strideDB,
-                            batchCount)
+                            dB, lddb, strideDB, batchCount)
                : mixed_gemm(handle, order, OP_T, ta ? OP_T : OP_N, N, K, M, al,
-                            dC, lddc, strideDC, true, A, lda, strideA,
-                            static_cast<float *>(dB), lddb, strideDB,
-                            batchCount);
+                            dC, lddc, strideDC, true, A, lda, strideA, dB,
+                            lddb, strideDB, batchCount);
       if (st != HCBLAS_STATUS_SUCCESS) return st;
     }
     dA = dB = nullptr;  // done; dC_in below
-  }
-  if (dtype == HCBLAS_R_16BF && (dA || dB)) {
-    // A is rows_a x cols_a in storage, B rows_b x cols_b.
-    const int ra = ta ? K : M, ca = ta ? M : K;
-    const int rb = tb ? N : K, cb = tb ? K : N;
-    const int64_t na = dB ? batch_span(order, ra, ca, lda, strideA, batchCount) : 0;
-    const int64_t nb = dA ? batch_span(order, rb, cb, ldb, strideB, batchCount) : 0;
-    if ((na && A == nullptr) || (nb && B == nullptr))
-      return HCBLAS_STATUS_INVALID_VALUE;
-    if (ensure_workspace(handle->adj_ws, sizeof(float) * (size_t)(na + nb) + 256) !=
-        hipSuccess)
-      return HCBLAS_STATUS_ALLOC_FAILED;
-    float *wa = static_cast<float *>(handle->adj_ws.ptr);
-    float *wb = wa + na;
-    if (hipSetDevice(handle->device) != hipSuccess)
-      return HCBLAS_STATUS_EXECUTION_FAILED;
-    if (na) {
-      bf16_to_f32_kernel<<<grid_for(na, handle->num_cus), 256, 0,
-                           handle->stream>>>(
-          na, static_cast<const uint16_t *>(A), wa);
-      Aw = wa;
-    }
-    if (nb) {
-      bf16_to_f32_kernel<<<grid_for(nb, handle->num_cus), 256, 0,
-                           handle->stream>>>(
-          nb, static_cast<const uint16_t *>(B), wb);
-      Bw = wb;
-    }
-    hcblasStatus_t st = launch_status();
-    if (st != HCBLAS_STATUS_SUCCESS) return st;
   }
   hcblasStatus_t st = HCBLAS_STATUS_SUCCESS;
   if (dA) {
     if (!ta)
       st = gemm_common(handle, wt, order, OP_N, tb ? OP_N : OP_T, M, K, N,
-                       alpha, dC, lddc, 0, strideDC, nullptr, Bw, ldb, 0,
+                       alpha, dC, lddc, 0, strideDC, nullptr, B, ldb, 0,
                        strideB, nullptr, zero, dA, ldda, 0, strideDA, nullptr,
                        batchCount);
     else
       st = gemm_common(handle, wt, order, tb ? OP_T : OP_N, OP_T, K, M, N,
-                       alpha, Bw, ldb, 0, strideB, nullptr, dC, lddc, 0,
+                       alpha, B, ldb, 0, strideB, nullptr, dC, lddc, 0,
                        strideDC, nullptr, zero, dA, ldda, 0, strideDA, nullptr,
                        batchCount);
     if (st != HCBLAS_STATUS_SUCCESS) return st;
@@ -997,12 +944,12 @@ hcblasStatus_t hcblas_gemm_strided_batched_backward(
   if (dB) {
     if (!tb)
       st = gemm_common(handle, wt, order, ta ? OP_N : OP_T, OP_N, K, N, M,
-                       alpha, Aw, lda, 0, strideA, nullptr, dC, lddc, 0,
+                       alpha, A, lda, 0, strideA, nullptr, dC, lddc, 0,
                        strideDC, nullptr, zero, dB, lddb, 0, strideDB, nullptr,
                        batchCount);
     else
       st = gemm_common(handle, wt, order, OP_T, ta ? OP_T : OP_N, N, K, M,
-                       alpha, dC, lddc, 0, strideDC, nullptr, Aw, lda, 0,
+                       alpha, dC, lddc, 0, strideDC, nullptr, A, lda, 0,
                        strideA, nullptr, zero, dB, lddb, 0, strideDB, nullptr,
                        batchCount);
     if (st != HCBLAS_STATUS_SUCCESS) return st;

@@ -37,10 +37,11 @@ hcblasStatus_t gemm_ozaki_crt(hcblasHandle_t h, const Canon &c, double alpha,
                               double beta, double *C, int64_t ldc,
                               int64_t cOff, int64_t strideC,
                               void *const *Cptrs, int64_t batch);
-// C = alpha * A' * B' (FP32 out, beta = 0) with A' FP32 and B' BF16
-// (a_f32) or A' BF16 and B' FP32; strided batch (gemm_mixed.hip).
+// C = alpha * A' * B' (beta = 0) with A' FP32 and B' BF16 (a_f32) or A'
+// BF16 and B' FP32, written as FP32 or (c_bf16) BF16; strided batch
+// (gemm_mixed.hip).
 hcblasStatus_t gemm_mixed_bf16(hcblasHandle_t h, const Canon &c, bool a_f32,
-                               float alpha, float *C, int64_t ldc,
+                               float alpha, void *C, bool c_bf16, int64_t ldc,
                                int64_t strideC, int64_t batch);
 hcblasStatus_t gemm_ozaki_crt_f32(hcblasHandle_t h, const Canon &c,
                                   float alpha, float beta, float *C,

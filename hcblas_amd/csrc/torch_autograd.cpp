@@ -37,7 +37,7 @@ void check(hcblasStatus_t st, const char *what) {
 }
 
 // One handle per (thread, device, stream): a handle's workspaces (split-K
-// partials, the FP32 operand copies of a BF16 backward) are used by the
+// partials, the transposed operand copies of a BF16 backward) are used by the
 // work it enqueues, so two streams never share one handle, and work on one
 // stream cannot overwrite a workspace that another stream is still reading.
 struct StreamKey {
@@ -159,8 +159,10 @@ class GemmBatchedFn : public torch::autograd::Function<GemmBatchedFn> {
     Tensor dC = grads[0].to(ot).contiguous();
     const bool need_a = ctx->needs_input_grad(0), need_b = ctx->needs_input_grad(1);
     const bool need_c = has_c && ctx->needs_input_grad(2);
-    Tensor dA = need_a ? torch::empty(A.sizes(), A.options().dtype(ot)) : Tensor();
-    Tensor dB = need_b ? torch::empty(B.sizes(), B.options().dtype(ot)) : Tensor();
+    // Gradients take the input's type (the library rounds BF16 adjoints
+    // once from the FP32 accumulation).
+    Tensor dA = need_a ? torch::empty(A.sizes(), A.options()) : Tensor();
+    Tensor dB = need_b ? torch::empty(B.sizes(), B.options()) : Tensor();
     Tensor dCin = need_c ? torch::empty_like(dC) : Tensor();
     if (need_a || need_b || need_c) {
       hcblasHandle_t h = handle_for(dC);
@@ -179,10 +181,6 @@ class GemmBatchedFn : public torch::autograd::Function<GemmBatchedFn> {
                 (int)s.batch),
             "hcblas_gemm_strided_batched_backward");
     }
-    // Gradients take the input's type (FP32 adjoints of BF16 inputs are
-    // rounded once here).
-    if (need_a) dA = dA.to(A.scalar_type());
-    if (need_b) dB = dB.to(B.scalar_type());
     return {dA, dB, dCin, Tensor(), Tensor(), Tensor(), Tensor()};
   }
 };

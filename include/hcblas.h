@@ -722,11 +722,12 @@ hcblasStatus_t hcblas_dgemm_ozaki_crt(hcblasHandle_t handle,
  *           op(B) = B^T);
  *   dC_in = beta * dC.
  * dtype HCBLAS_R_32F / HCBLAS_R_64F: every matrix of that type, alpha / beta
- * point to float / double. HCBLAS_R_16BF: A and B are bfloat16, dC, dA, dB
- * and dC_in are float (the forward's accumulation type), alpha / beta are
- * float. dA, dB or dCin may be NULL to skip that adjoint. op = C is treated
- * as T for these real types. The handle's math mode applies to the adjoint
- * GEMMs as to any GEMM. */
+ * point to float / double. HCBLAS_R_16BF: A, B, dA and dB are bfloat16 (dA
+ * and dB accumulated in FP32 from the exact BF16 split of dC and rounded
+ * once to nearest even), dC and dC_in are float (the forward's accumulation
+ * type), alpha / beta are float. dA, dB or dCin may be NULL to skip that
+ * adjoint. op = C is treated as T for these real types. The handle's math
+ * mode applies to the FP32 / FP64 adjoint GEMMs as to any GEMM. */
 hcblasStatus_t hcblas_gemm_strided_batched_backward(
     hcblasHandle_t handle, hcblasDatatype_t dtype, hcblasOrder_t order,
     hcblasOperation_t typeA, hcblasOperation_t typeB, int M, int N, int K,
