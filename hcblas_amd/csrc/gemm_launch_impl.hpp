@@ -123,7 +123,15 @@ hcblasStatus_t launch_gemm(hipStream_t stream, int num_cus, GemmParams p,
   // 16-bit types (more waves per CU to cover the load latency when there
   // are only about as many tiles as CUs: 1024^3 SGEMM 0.0376 ms against
   // 0.0382 with 4 waves, the conv-net shape set 24.6 TF/s against 23.8),
-  // 4 waves of 32x32 for FP64.
+  // 4 waves of 32x32 for FP64. FP32 with K >= 512 takes 128-wide K steps:
+  // a 32-wide step is 16 MFMAs per wave, too short to cover the global loads
+  // issued one step ahead (1024^3 kernel 27.5 us against 31.6 with BK = 32,
+  // 29.5 with 64); shorter K keeps 32 so the last step pads little.
+  if constexpr (E == Elem::F32) {
+    if (p.K >= 512)
+      return launch_tiles<E, OutT, 64, 64, 128, 2, 4>(stream, p, a_k, b_k,
+                                                      vload, batch);
+  }
   if constexpr (E != Elem::F64)
     return launch_tiles<E, OutT, 64, 64, TileCfg<E>::BK_SMALL, 2, 4>(
         stream, p, a_k, b_k, vload, batch);
