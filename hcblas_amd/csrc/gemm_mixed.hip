@@ -301,15 +301,70 @@ __global__ __launch_bounds__(256) void transpose_kernel(
   }
 }
 
+// The same copy with 16-byte accesses on both sides, for whole tiles of 64
+// rows x 256 bytes (rows % 64 == 0, cols a multiple of 256 bytes, every
+// base, leading dimension and batch stride a whole number of vectors): a
+// thread loads four 16-byte row pieces into LDS (row stride one element
+// more than the tile width), then gathers four 16-byte output pieces down
+// the LDS columns. One element per lane each way moved 4.4 TB/s (FP32) and
+// 3.7 TB/s (BF16) on the differentiation configuration's copies.
+template <typename T>
+__global__ __launch_bounds__(256) void transpose_vec_kernel(
+    const T *__restrict__ in, int64_t ldi, int64_t si, T *__restrict__ out,
+    int64_t ldo, int64_t so) {
+  constexpr int VE = 16 / (int)sizeof(T);  // elements per vector
+  constexpr int TC = 256 / (int)sizeof(T);  // tile columns (input)
+  constexpr int TR = 64;                    // tile rows (input)
+  __shared__ T t[TR][TC + 1];
+  const int r0 = blockIdx.y * TR, c0 = blockIdx.x * TC;
+  const T *src = in + (int64_t)blockIdx.z * si;
+  T *dst = out + (int64_t)blockIdx.z * so;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int v = threadIdx.x + 256 * i;
+    const int r = v / (TC / VE), vc = (v % (TC / VE)) * VE;
+    const vec4u w = *reinterpret_cast<const vec4u *>(
+        src + (int64_t)(r0 + r) * ldi + c0 + vc);
+    T e[VE];
+    __builtin_memcpy(e, &w, 16);
+#pragma unroll
+    for (int j = 0; j < VE; ++j) t[r][vc + j] = e[j];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int v = threadIdx.x + 256 * i;
+    const int c = v / (TR / VE), rv = (v % (TR / VE)) * VE;
+    T e[VE];
+#pragma unroll
+    for (int j = 0; j < VE; ++j) e[j] = t[rv + j][c];
+    vec4u w;
+    __builtin_memcpy(&w, e, 16);
+    *reinterpret_cast<vec4u *>(dst + (int64_t)(c0 + c) * ldo + r0 + rv) = w;
+  }
+}
+
 template <typename T>
 void launch_transpose(hipStream_t s, int rows, int cols, const T *in,
                       int64_t ldi, int64_t si, T *out, int64_t ldo, int64_t so,
                       int64_t batch) {
+  constexpr int VE = 16 / (int)sizeof(T), TC = 256 / (int)sizeof(T);
+  const bool vec = rows % 64 == 0 && cols % TC == 0 && ldi % VE == 0 &&
+                   ldo % VE == 0 && si % VE == 0 && so % VE == 0 &&
+                   reinterpret_cast<uintptr_t>(in) % 16 == 0 &&
+                   reinterpret_cast<uintptr_t>(out) % 16 == 0;
   for (int64_t b0 = 0; b0 < batch; b0 += 65535) {
     const int64_t nb = batch - b0 < 65535 ? batch - b0 : 65535;
-    dim3 g((unsigned)cdiv(cols, 64), (unsigned)cdiv(rows, 64), (unsigned)nb);
-    transpose_kernel<T><<<g, 256, 0, s>>>(rows, cols, in + b0 * si, ldi, si,
-                                          out + b0 * so, ldo, so);
+    if (vec) {
+      dim3 g((unsigned)(cols / TC), (unsigned)(rows / 64), (unsigned)nb);
+      transpose_vec_kernel<T><<<g, 256, 0, s>>>(in + b0 * si, ldi, si,
+                                                out + b0 * so, ldo, so);
+    } else {
+      dim3 g((unsigned)cdiv(cols, 64), (unsigned)cdiv(rows, 64),
+             (unsigned)nb);
+      transpose_kernel<T><<<g, 256, 0, s>>>(rows, cols, in + b0 * si, ldi,
+                                            si, out + b0 * so, ldo, so);
+    }
   }
 }
 
