@@ -86,7 +86,7 @@ struct TileStage {
 template <typename T, int RT, int KT>
 struct TileRK {
   T v[RT][KT + 1];
-  template <bool KC>
+  template <bool KC, bool NTC>
   __device__ __forceinline__ void load_as(const T *__restrict__ X, int64_t ld,
                                           bool vec, int R, int K, int r0,
                                           int k0) {
@@ -107,7 +107,13 @@ struct TileRK {
       T t[VE];
       const bool full = KC ? (r < R && k + VE <= K) : (k < K && r + VE <= R);
       if (vec && full) {
-        const vec4u w = *reinterpret_cast<const vec4u *>(src);
+        // The operand is read once: non-temporal loads, except for columns
+        // along r when NTC is false (measured slower for the residue
+        // kernel's 32-element pieces).
+        const vec4u w =
+            (KC || NTC)
+                ? __builtin_nontemporal_load(reinterpret_cast<const vec4u *>(src))
+                : *reinterpret_cast<const vec4u *>(src);
         __builtin_memcpy(t, &w, 16);
       } else {
 #pragma unroll
@@ -125,13 +131,14 @@ struct TileRK {
       }
     }
   }
+  template <bool NTC = true>
   __device__ __forceinline__ void load(const T *__restrict__ X, int64_t ld,
                                        bool kcont, bool vec, int R, int K,
                                        int r0, int k0) {
     if (kcont)
-      load_as<true>(X, ld, vec, R, K, r0, k0);
+      load_as<true, NTC>(X, ld, vec, R, K, r0, k0);
     else
-      load_as<false>(X, ld, vec, R, K, r0, k0);
+      load_as<false, NTC>(X, ld, vec, R, K, r0, k0);
     __syncthreads();
   }
 };
@@ -672,7 +679,7 @@ struct ResidueWeights {
 };
 
 // Residue planes of the 16 integers of one row segment: one 16-byte store
-// per modulus plane (plane t of the row at row + t * Kp). Each stored byte is
+// per modulus plane (plane t's vector at row + t * pstride). Each stored byte is
 // the symmetric residue of v modulo m_t, in [-(m-1)/2, (m-1)/2] for odd m
 // and the two's-complement low byte of v for m = 256. For odd m, s (the
 // weighted byte sum, s < 2^19) is exact in float, rint(s * (1/m)) is the
@@ -713,37 +720,35 @@ __device__ __forceinline__ void write_residues16(const ResidueIn<WIDE> &in,
     w.y = pack_bytes(q[4], q[5], q[6], q[7]);
     w.z = pack_bytes(q[8], q[9], q[10], q[11]);
     w.w = pack_bytes(q[12], q[13], q[14], q[15]);
-    *reinterpret_cast<vec4u *>(row + (int64_t)T * pstride) = w;
+    __builtin_nontemporal_store(w, reinterpret_cast<vec4u *>(row + (int64_t)T * pstride));
     write_residues16<NMOD, WIDE, T + 1>(in, row, pstride);
   }
 }
 
-// Residue planes P[r][t][0..Kp) of v = rint(X(r, k) * 2^(bits - e_r)) (Kp a
-// multiple of 128; zeros past K). A workgroup stages a 32 x 128 tile through
-// LDS, reading along the source's contiguous dimension; each thread then
-// converts 16 consecutive k of one row and writes one 16-byte vector per
-// plane, so eight threads write a whole 128-byte line of each plane row
-// (a 64-wide tile wrote half lines, which the memory merges less well).
+// Residue planes of v = rint(X(r, k) * 2^(bits - e_r)) over k < Kp (Kp a
+// multiple of 128; zeros past K), in MFMA fragment order: plane t starts at
+// P + t * pstride (pstride = Rpad * Kp, Rpad a multiple of 256), and the
+// 32 x 32 block (r / 32, k / 32) is one 1 KiB run holding the 16 bytes of
+// (r, 16-k group g) at lane (r % 32) + 32 * (g % 2) -- the operand layout of
+// v_mfma_i32_32x32x32_i8, so a wave loads a fragment with one coalesced
+// 16-byte load per lane. Rows from R to Rpad are not written. A workgroup
+// stages a 32 x 128 tile through LDS, reading along the source's contiguous
+// dimension; each thread then converts 16 consecutive k of one row and
+// writes one 16-byte vector per plane, so a wave writes two whole 1 KiB
+// runs of each plane.
 // FP32 input: bits - e_r keeps |v| <= 2^30, and the scaling and rounding are
 // exact in float (values below 2^-126 round to 0 either way); FP64 input:
 // |v| <= 2^62, split exactly into 32-bit halves in double.
-//
-// With fstride != 0 the planes are written in MFMA fragment order instead:
-// plane t starts at P + t * fstride (fstride = Rpad * Kp, Rpad a multiple
-// of 256), and the 32 x 32 block (r / 32, k / 32) is one 1 KiB run holding
-// the 16 bytes of (r, 16-k group g) at lane (r % 32) + 32 * (g % 2) -- the
-// operand layout of v_mfma_i32_32x32x32_i8, so a wave loads a fragment with
-// one coalesced 16-byte load per lane. Rows from R to Rpad are not written.
 template <typename T, int NMOD>
 __global__ __launch_bounds__(256) void crt_residue_kernel(
     int R, int K, const T *__restrict__ X, int64_t ld, bool kcont, bool vec,
     const int *__restrict__ exps, int bits, int8_t *__restrict__ P,
-    int64_t Kp, int64_t fstride) {
+    int64_t Kp, int64_t pstride) {
   constexpr bool WIDE = sizeof(T) == 8;
   __shared__ TileRK<T, 32, 128> st;
   const int r0 = blockIdx.y * 32, k0 = blockIdx.x * 128;
   const int tid = threadIdx.x;
-  st.load(X, ld, kcont, vec, R, K, r0, k0);
+  st.template load<false>(X, ld, kcont, vec, R, K, r0, k0);
   const int row = tid >> 3, kq = (tid & 7) * 16;
   const int r = r0 + row;
   if (r >= R) return;
@@ -766,23 +771,19 @@ __global__ __launch_bounds__(256) void crt_residue_kernel(
       in.lo[j] = (uint32_t)(int)v + (1u << 30);
     }
   }
-  if (fstride == 0) {
-    write_residues16<NMOD, WIDE>(in, P + (int64_t)r * NMOD * Kp + k0 + kq, Kp);
-  } else {
-    const int k = k0 + kq;
-    const int64_t blk = (int64_t)(r >> 5) * (Kp >> 5) + (k >> 5);
-    const int lane = (r & 31) + 32 * ((k >> 4) & 1);
-    write_residues16<NMOD, WIDE>(in, P + blk * 1024 + lane * 16, fstride);
-  }
+  const int k = k0 + kq;
+  const int64_t blk = (int64_t)(r >> 5) * (Kp >> 5) + (k >> 5);
+  const int lane = (r & 31) + 32 * ((k >> 4) & 1);
+  write_residues16<NMOD, WIDE>(in, P + blk * 1024 + lane * 16, pstride);
 }
 
 struct CrtParams {
   int M, N;
   int64_t Kp;
   int nmod;
-  const int8_t *Ap, *Bp;  // residue planes: A rows of nmod * Kp, B in
-                          // fragment order (plane t at Bp + t * bstride)
-  int64_t bstride;
+  const int8_t *Ap, *Bp;  // residue planes in fragment order (plane t at
+                          // Ap + t * astride, Bp + t * bstride)
+  int64_t astride, bstride;
   uint8_t *R;             // R[t][n][m]: (A'B')(m, n) mod m_t in [0, m_t)
   int tiles_m, tiles_n;
 };
@@ -853,10 +854,9 @@ __device__ __forceinline__ void crt_store_tile(const AccArr &acc, int mod,
 }
 
 // Main loop of the residue GEMM: a 256 x 256 tile of one modulus' product
-// over INT8 planes whose length Kp is a multiple of 128 (zero padded), 4
+// over INT8 planes whose length Kp is a multiple of 256 (zero padded), 4
 // waves of 128 x 128 (4 x 4 blocks of v_mfma_i32_32x32x32_i8), one wave per
-// SIMD. A comes from k-contiguous rows (row m at Ap + m * lda), B from
-// fragment-ordered planes (crt_residue_kernel with fstride != 0).
+// SIMD. Both operands come from fragment-ordered planes (crt_residue_kernel).
 //
 // On gfx950 the LDS moves about 100 bytes per clock per CU for
 // ds_write_b128 and not much more for reads, so a 256 x 256 x 128 step
@@ -876,18 +876,17 @@ __device__ __forceinline__ void crt_store_tile(const AccArr &acc, int mod,
 //     arrive; a second B fragment set as well spills registers);
 //   - the one barrier of a step comes before the MFMAs of its last chunk,
 //     followed by the A reads of the next step's first chunk.
-// Rows past M load row m0 (inside the planes); B rows past N are inside the
-// padded planes. Either only feeds outputs the epilogue never stores.
+// Rows past M or N are inside the padded planes (never written, so any
+// bytes); they only feed outputs the epilogue never stores.
 constexpr int kCrtBM = 256, kCrtBK = 128, kCrtRowB = kCrtBK + 16;
 constexpr int kCrtBuf = kCrtBM * kCrtRowB;  // the A tile of one step
 constexpr int kCrtLds = 2 * kCrtBuf;
 
 using CrtAcc = intx16[4][4];
 
-__device__ __forceinline__ void crt_mainloop(const int8_t *__restrict__ Ap,
-                                             int64_t lda,
+__device__ __forceinline__ void crt_mainloop(const int8_t *__restrict__ Af,
                                              const int8_t *__restrict__ Bf,
-                                             int M, int Kp, int m0, int n0,
+                                             int Kp, int m0, int n0,
                                              char *lds, CrtAcc &acc) {
   constexpr int NTH = 256, PT = kCrtBM * (kCrtBK / 16) / NTH;  // 8
   const int tid = threadIdx.x;
@@ -898,23 +897,21 @@ __device__ __forceinline__ void crt_mainloop(const int8_t *__restrict__ Ap,
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = intx16{};
-  // A loader: vector v = tid + NTH * i is row v / 8, bytes (v % 8) * 16.
-  const int lr = tid >> 3, lk = (tid & 7) * 16;
-  const int8_t *ga[PT];
-#pragma unroll
-  for (int i = 0; i < PT; ++i) {
-    const int ra = m0 + lr + 32 * i;
-    ga[i] = Ap + (int64_t)(ra < M ? ra : m0) * lda + lk;
-  }
+  // A loader: a K step of the tile is 8 row blocks x 4 chunks of 1 KiB
+  // fragments; load i of wave w is fragment (row block i, chunk w), lane l
+  // holding row 32 i + l % 32, bytes 32 w + 16 (l / 32) of the step.
+  const int64_t fstep = (int64_t)Kp * 32;  // bytes per row block of a plane
+  const int8_t *ga =
+      Af + (int64_t)(m0 >> 5) * fstep + wave * 1024 + lane * 16;
+  const int soff = lrow * kCrtRowB + wave * 32 + lgrp * 16;
   // Two staging sets: tile j of A goes through set j % 2 and is loaded
   // seven chunks before the chunk that stores it.
   vec4u sa[2][PT];
-  auto gload = [&](vec4u(&st)[PT], int k0) {
+  auto gload = [&](vec4u(&st)[PT], int k0) {  // k0: byte offset of the step
 #pragma unroll
     for (int i = 0; i < PT; ++i)
-      st[i] = *reinterpret_cast<const vec4u *>(ga[i] + k0);
+      st[i] = *reinterpret_cast<const vec4u *>(ga + i * fstep + k0 * 32);
   };
-  const int soff = lr * kCrtRowB + lk;
   auto store_a = [&](const vec4u(&st)[PT], char *b) {
 #pragma unroll
     for (int i = 0; i < PT; ++i)
@@ -952,12 +949,28 @@ __device__ __forceinline__ void crt_mainloop(const int8_t *__restrict__ Ap,
   };
   const int nk = Kp / kCrtBK;
   if (nk == 0) return;
+  // The prologue issues its loads in the order the loop does (tile 1, B
+  // chunk 0, tile 2, B chunks 1-3), unconditionally (tiles past the planes'
+  // length reload tile 0 and are never stored), so at the loop head every
+  // register has as many younger loads outstanding on entry as on the back
+  // edge. The wait counts the compiler derives for the loop from the merge
+  // of the two paths are then the loop's own; a skipped load on either path
+  // would make it wait for loads issued a chunk earlier.
   gload(sa[0], 0);
-#pragma unroll
-  for (int c = 0; c < 4; ++c) bload(fbX, 0, c);
+  __builtin_amdgcn_sched_barrier(0);
+  gload(sa[1], nk > 1 ? kCrtBK : 0);
+  __builtin_amdgcn_sched_barrier(0);
+  bload(fbX, 0, 0);
+  __builtin_amdgcn_sched_barrier(0);
   store_a(sa[0], buf(0));
-  if (nk > 1) gload(sa[1], kCrtBK);
-  if (nk > 2) gload(sa[0], 2 * kCrtBK);
+  __builtin_amdgcn_sched_barrier(0);
+  gload(sa[0], nk > 2 ? 2 * kCrtBK : 0);
+#pragma unroll
+  for (int c = 1; c < 4; ++c) {
+    __builtin_amdgcn_sched_barrier(0);
+    bload(fbX, 0, c);
+  }
+  __builtin_amdgcn_sched_barrier(0);
   __syncthreads();
   frags(fa[0], buf(0), 0);
   // Issue order inside a chunk: one MFMA, then one of the chunk's memory
@@ -975,60 +988,42 @@ __device__ __forceinline__ void crt_mainloop(const int8_t *__restrict__ Ap,
     __builtin_amdgcn_sched_barrier(0);
   };
   // Step kt with B fragments fb (set kt % 2) and A staging set sn (holding
-  // tile kt + 1, refilled with tile kt + 3). Steady: every later tile
-  // exists, so no condition splits a chunk's scheduling region.
+  // tile kt + 1, refilled with tile kt + 3). Every step runs this schedule:
+  // loads of steps past the last one reload the last step's data, which is
+  // never used (the tile they stage goes to the LDS buffer no later step
+  // reads), so no condition splits a chunk's scheduling region.
+  const int last = nk - 1;
   auto steady = [&](int kt, vec4u(&fb)[4][4], vec4u(&sn)[PT]) {
     const char *cur = buf(kt);
     char *nxt = buf(kt + 1);
+    const int kb = kt + 1 < last ? kt + 1 : last;
+    const int ka = kt + 3 < last ? kt + 3 : last;
     frags(fa[1], cur, 1);
     store_a(sn, nxt);
     mfmas(fa[0], fb[0]);
-    bload(fb, kt + 1, 0);
+    bload(fb, kb, 0);
     interleave(4, 8, 4);
     frags(fa[0], cur, 2);
-    gload(sn, (kt + 3) * kCrtBK);
+    gload(sn, ka * kCrtBK);
     mfmas(fa[1], fb[1]);
-    bload(fb, kt + 1, 1);
+    bload(fb, kb, 1);
     interleave(4, 0, 12);
     frags(fa[1], cur, 3);
     mfmas(fa[0], fb[2]);
-    bload(fb, kt + 1, 2);
+    bload(fb, kb, 2);
     interleave(4, 0, 4);
     __syncthreads();
     frags(fa[0], nxt, 0);
     mfmas(fa[1], fb[3]);
-    bload(fb, kt + 1, 3);
+    bload(fb, kb, 3);
     interleave(4, 0, 4);
   };
-  auto tail = [&](int kt, vec4u(&fb)[4][4], vec4u(&sn)[PT]) {
-    const char *cur = buf(kt);
-    char *nxt = buf(kt + 1);
-    const bool more = kt + 1 < nk, more2 = more;
-    frags(fa[1], cur, 1);
-    if (more) store_a(sn, nxt);
-    mfmas(fa[0], fb[0]);
-    if (more2) bload(fb, kt + 1, 0);
-    frags(fa[0], cur, 2);
-    if (kt + 3 < nk) gload(sn, (kt + 3) * kCrtBK);
-    mfmas(fa[1], fb[1]);
-    if (more2) bload(fb, kt + 1, 1);
-    frags(fa[1], cur, 3);
-    mfmas(fa[0], fb[2]);
-    if (more2) bload(fb, kt + 1, 2);
-    __syncthreads();
-    if (more) frags(fa[0], nxt, 0);
-    mfmas(fa[1], fb[3]);
-    if (more2) bload(fb, kt + 1, 3);
-  };
   int kt = 0;
-  for (; kt + 4 < nk; kt += 2) {
+  do {  // nk is even and positive
     steady(kt, fbX, sa[1]);
     steady(kt + 1, fbX, sa[0]);
-  }
-  for (; kt < nk; kt += 2) {
-    tail(kt, fbX, sa[1]);
-    if (kt + 1 < nk) tail(kt + 1, fbX, sa[0]);
-  }
+    kt += 2;
+  } while (kt < nk);
 }
 
 // One INT8 GEMM per modulus (blockIdx.z = t) with the 4-wave main loop.
@@ -1039,8 +1034,8 @@ __global__ __launch_bounds__(256) void crt_gemm4_kernel(CrtParams p) {
   int tm, tn;
   tile_coords(blockIdx.x, p.tiles_m, p.tiles_n, tm, tn);
   CrtAcc acc;
-  crt_mainloop(p.Ap + t * p.Kp, (int64_t)p.nmod * p.Kp, p.Bp + t * p.bstride,
-               p.M, (int)p.Kp, tm * kCrtBM, tn * kCrtBM, lds, acc);
+  crt_mainloop(p.Ap + t * p.astride, p.Bp + t * p.bstride, (int)p.Kp,
+               tm * kCrtBM, tn * kCrtBM, lds, acc);
   crt_store_tile<kCrtBM, kCrtBM, 2, 2, kCrtLds>(
       acc, mod, p.R + (int64_t)t * p.M * p.N, p.M, p.N, tm * kCrtBM,
       tn * kCrtBM, lds);
@@ -1230,7 +1225,7 @@ __global__ __launch_bounds__(256) void crt_combine_kernel(
       for (int q = 0; q < 4 / VE; ++q) {
         vec4u cv;
         __builtin_memcpy(&cv, o + q * VE, 16);
-        *reinterpret_cast<vec4u *>(dst + q * VE) = cv;
+        __builtin_nontemporal_store(cv, reinterpret_cast<vec4u *>(dst + q * VE));
       }
     } else {
 #pragma unroll
@@ -1248,11 +1243,11 @@ __global__ __launch_bounds__(256) void crt_combine_kernel(
 template <int NMOD, typename T>
 void launch_crt_residues(hipStream_t s, int R, int K, const T *X, int64_t ld,
                          bool kcont, const int *exps, int bits, int8_t *P,
-                         int64_t Kp, int64_t fstride) {
+                         int64_t Kp, int64_t pstride) {
   // Kp is a multiple of 128 (the INT8 GEMM's K step).
   dim3 g((unsigned)(Kp / 128), (unsigned)cdiv(R, 32));
   crt_residue_kernel<T, NMOD><<<g, 256, 0, s>>>(
-      R, K, X, ld, kcont, vec_aligned(X, ld), exps, bits, P, Kp, fstride);
+      R, K, X, ld, kcont, vec_aligned(X, ld), exps, bits, P, Kp, pstride);
 }
 template <int NMOD, typename OutT>
 void launch_crt_combine(hipStream_t s, int M, int N, const uint8_t *R,
@@ -1470,7 +1465,7 @@ hcblasStatus_t gemm_ozaki_crt(hcblasHandle_t h, const Canon &c, double alpha,
   if (alpha == 0.0 || c.K == 0)
     return gemm_real<Elem::F64, double>(h, c, alpha, beta, C, ldc, cOff,
                                         strideC, Cptrs, batch);
-  constexpr int BK = kCrtBK;
+  constexpr int BK = 2 * kCrtBK;  // the main loop takes K steps in pairs
   // INT32 exactness of one modulus product: kc * 128 * 128 < 2^31.
   constexpr int64_t kmax = 65536;
   const int64_t kchunk = std::min<int64_t>(cdiv(c.K, BK) * BK, kmax);
@@ -1480,9 +1475,10 @@ hcblasStatus_t gemm_ozaki_crt(hcblasHandle_t h, const Canon &c, double alpha,
                        ? h->ozaki_moduli
                        : std::max(crt_auto_moduli((int)kchunk),
                                   crt_auto_moduli((int)(c.K % kchunk ? c.K % kchunk : kchunk)));
-  const size_t a_bytes = (size_t)nmax * c.M * Kp;
-  // B planes in fragment order over N rounded up to the 256-column tile.
+  // Planes in fragment order over M and N rounded up to the 256-wide tile.
+  const int64_t Mpad = cdiv(c.M, kCrtBM) * kCrtBM;
   const int64_t Npad = cdiv(c.N, kCrtBM) * kCrtBM;
+  const size_t a_bytes = (size_t)nmax * Mpad * Kp;
   const size_t b_bytes = (size_t)nmax * Npad * Kp;
   const size_t r_bytes = (size_t)nmax * c.M * c.N;
   const size_t e_bytes = sizeof(int) * ((size_t)c.M + c.N);
@@ -1518,7 +1514,7 @@ hcblasStatus_t gemm_ozaki_crt(hcblasHandle_t h, const Canon &c, double alpha,
       launch_row_exp(h->stream, c.N, kc, B, c.ldb, c.b_k, eb);
       if (!with_nmod<ResidueLaunch>(nmod, h->stream, c.M, kc, A, c.lda,
                                     c.a_k, (const int *)ea, bits, Ap, Kp,
-                                    (int64_t)0) ||
+                                    Mpad * Kp) ||
           !with_nmod<ResidueLaunch>(nmod, h->stream, c.N, kc, B, c.ldb,
                                     c.b_k, (const int *)eb, bits, Bp, Kp,
                                     Npad * Kp))
@@ -1532,6 +1528,7 @@ hcblasStatus_t gemm_ozaki_crt(hcblasHandle_t h, const Canon &c, double alpha,
       p.nmod = nmod;
       p.Ap = Ap;
       p.Bp = Bp;
+      p.astride = Mpad * Kp;
       p.bstride = Npad * Kp;
       p.R = R;
       p.tiles_m = (int)cdiv(c.M, kCrtBM);
@@ -1580,15 +1577,17 @@ hcblasStatus_t gemm_ozaki_crt_f32(hcblasHandle_t h, const Canon &c,
   if (alpha == 0.f || c.K == 0)
     return gemm_real<Elem::F32, float>(h, c, alpha, beta, C, ldc, cOff,
                                        strideC, Cptrs, batch);
-  constexpr int BK = kCrtBK;
+  constexpr int BK = 2 * kCrtBK;  // the main loop takes K steps in pairs
   constexpr int64_t kmax = 65536;  // INT32 exactness: kc * 128 * 128 < 2^31
   const int64_t kchunk = std::min<int64_t>(cdiv(c.K, BK) * BK, kmax);
   const int64_t Kp = kchunk;
   const int nmod = h->ozaki_moduli ? h->ozaki_moduli : 8;
   const int bb = crt_beta_f32(nmod);
   const int pa = (int)cdiv(kchunk, 64), pb = pa;  // 64-wide k blocks
-  const size_t a_bytes = (size_t)nmod * c.M * Kp;
-  const int64_t Npad = cdiv(c.N, kCrtBM) * kCrtBM;  // B in fragment order
+  // Planes in fragment order over M and N rounded up to the 256-wide tile.
+  const int64_t Mpad = cdiv(c.M, kCrtBM) * kCrtBM;
+  const int64_t Npad = cdiv(c.N, kCrtBM) * kCrtBM;
+  const size_t a_bytes = (size_t)nmod * Mpad * Kp;
   const size_t b_bytes = (size_t)nmod * Npad * Kp;
   const size_t r_bytes = (size_t)nmod * c.M * c.N;
   const size_t e_bytes = sizeof(int) * ((size_t)c.M + c.N);
@@ -1632,7 +1631,7 @@ hcblasStatus_t gemm_ozaki_crt_f32(hcblasHandle_t h, const Canon &c,
       norms(c.N, kc, B, c.ldb, c.b_k, partb, eb);
       if (!with_nmod<ResidueLaunch>(nmod, h->stream, c.M, kc, A, c.lda,
                                     c.a_k, (const int *)ea, 0, Ap, Kp,
-                                    (int64_t)0) ||
+                                    Mpad * Kp) ||
           !with_nmod<ResidueLaunch>(nmod, h->stream, c.N, kc, B, c.ldb,
                                     c.b_k, (const int *)eb, 0, Bp, Kp,
                                     Npad * Kp))
@@ -1646,6 +1645,7 @@ hcblasStatus_t gemm_ozaki_crt_f32(hcblasHandle_t h, const Canon &c,
       p.nmod = nmod;
       p.Ap = Ap;
       p.Bp = Bp;
+      p.astride = Mpad * Kp;
       p.bstride = Npad * Kp;
       p.R = R;
       p.tiles_m = (int)cdiv(c.M, kCrtBM);
