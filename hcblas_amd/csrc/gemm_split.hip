@@ -324,16 +324,17 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_split3_kernel(
     for (int j = 0; j < FN; ++j) blk[i][j] = tot[i][j] = floatx16{};
 
   const int nsteps = p.nkb / SKB;  // nkb is a multiple of SKB
-  LA la;
-  LB lb;
-  auto load = [&](int s) {
-    la.load(p.Ap, p.M, m0, s * SKB);
-    lb.load(p.Bp, p.N, n0, s * SKB);
+  // Two register sets: step j is staged in set j % 2.
+  LA la[2];
+  LB lb[2];
+  auto load = [&](int set, int s) {
+    la[set].load(p.Ap, p.M, m0, s * SKB);
+    lb[set].load(p.Bp, p.N, n0, s * SKB);
   };
   auto buf = [&](int s) { return lds + (s & 1) * (A_BYTES + B_BYTES); };
-  auto store = [&](int s) {
-    la.store(buf(s));
-    lb.store(buf(s) + A_BYTES);
+  auto store = [&](int set, int s) {
+    la[set].store(buf(s));
+    lb[set].store(buf(s) + A_BYTES);
   };
   auto compute = [&](const char *cur) {
     const char *As = cur + (wm * WTM + l32) * ROWB + half * 16;
@@ -383,20 +384,36 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_split3_kernel(
       }
   };
 
-  // Registers hold step s+1 while step s computes; they are written to the
-  // other LDS buffer at the top of the step and refilled with step s+2.
-  load(0);
-  store(0);
-  if (nsteps > 1) load(1);
+  // While step s computes, the registers hold steps s + 1 and s + 2; step
+  // s + 1 is written to the other LDS buffer at the top of the step and its
+  // set is refilled with step s + 3, which has two steps to arrive. Loads
+  // past the last step reload it and stores past it go to the buffer no
+  // later step reads, so every step issues the same loads: the wait counts
+  // the compiler derives are then the same on every path into the loop.
+  const int last = nsteps - 1;
+  auto clampk = [&](int s) { return s < last ? s : last; };
+  load(0, 0);
+  __builtin_amdgcn_sched_barrier(0);
+  load(1, clampk(1));
+  __builtin_amdgcn_sched_barrier(0);
+  store(0, 0);
+  __builtin_amdgcn_sched_barrier(0);
+  load(0, clampk(2));
   __syncthreads();
-  for (int s = 0; s < nsteps; ++s) {
-    if (s + 1 < nsteps) store(s + 1);
-    if (s + 2 < nsteps) load(s + 2);
+  auto step = [&](int s, int set) {  // set = (s + 1) % 2
+    store(set, s + 1);
+    load(set, clampk(s + 3));
     __builtin_amdgcn_sched_barrier(0);
     compute(buf(s));
     if ((s + 1) % kSplit3Fold == 0) fold();
     __syncthreads();
+  };
+  int s = 0;
+  for (; s + 1 < nsteps; s += 2) {
+    step(s, 1);
+    step(s + 1, 0);
   }
+  if (s < nsteps) step(s, 1);
   fold();
 
 #pragma unroll
