@@ -872,16 +872,19 @@ __device__ __forceinline__ void crt_store_tile(const AccArr &acc, int mod,
 
 // Main loop of the residue GEMM: a 256 x 256 tile of one modulus' product
 // over INT8 planes whose length Kp is a multiple of 256 (zero padded), 4
-// waves of 128 x 128 (4 x 4 blocks of v_mfma_i32_32x32x32_i8), one wave per
-// SIMD. Both operands come from fragment-ordered planes (crt_residue_kernel).
+// waves, one per SIMD, each with 16 blocks of v_mfma_i32_32x32x32_i8 (256
+// accumulator registers). Both operands come from fragment-ordered planes
+// (crt_residue_kernel).
 //
-// On gfx950 the LDS moves about 100 bytes per clock per CU for
-// ds_write_b128 and not much more for reads, so a 256 x 256 x 128 step
-// staging both operands (64 KiB written, 128 KiB read by 4 waves) keeps the
-// LDS about as busy as the 2048 MFMA cycles per SIMD of the step. Here only
-// A goes through LDS (32 KiB written, 64 KiB read); each wave loads its B
-// fragments itself, one 1 KiB coalesced load per fragment, from planes
-// laid out in MFMA operand order. Latency is hidden inside the wave:
+// A goes through LDS; each wave loads its B fragments itself, one 1 KiB
+// coalesced load per fragment. The waves split the tile 1 x 4: each takes
+// all 256 rows and 64 columns, so every B fragment is loaded by one wave
+// only, and a 128-deep K step moves 64 KiB through the vector memory path
+// (32 KiB of A staging, 32 KiB of B) and reads 128 KiB from LDS. A 2 x 2
+// split (128 x 128 per wave) halves the LDS reads but loads every B
+// fragment twice (96 KiB per step); measured at 8192^3 it is 3% slower,
+// with the texture data path busy 85% of the time. Latency is hidden
+// inside the wave:
 //   - the A fragments of chunk c + 1 are read while the MFMAs of chunk c
 //     run (two A fragment sets);
 //   - the B fragments of chunk c of step kt + 1 are loaded right after the
@@ -890,30 +893,37 @@ __device__ __forceinline__ void crt_store_tile(const AccArr &acc, int mod,
 //     the tile of step kt + 1 is written to the other LDS buffer during
 //     chunk 0 of step kt and its set is refilled with step kt + 3 during
 //     chunk 1 (2-3% faster than one set, whose loads had three chunks to
-//     arrive; a second B fragment set as well spills registers);
+//     arrive);
 //   - the one barrier of a step comes before the MFMAs of its last chunk,
 //     followed by the A reads of the next step's first chunk.
 // Rows past M or N are inside the padded planes (never written, so any
 // bytes); they only feed outputs the epilogue never stores.
+#ifndef HCB_CRT_WN
+#define HCB_CRT_WN 4  // waves along n (4: 1 x 4, 2: 2 x 2)
+#endif
 constexpr int kCrtBM = 256, kCrtBK = 128, kCrtRowB = kCrtBK + 16;
 constexpr int kCrtBuf = kCrtBM * kCrtRowB;  // the A tile of one step
 constexpr int kCrtLds = 2 * kCrtBuf;
+// Waves of the tile: kCrtWM x kCrtWN, each with FM x FN 32 x 32 blocks.
+constexpr int kCrtWN = HCB_CRT_WN, kCrtWM = 4 / kCrtWN;
+constexpr int kCrtFM = 8 / kCrtWM, kCrtFN = 8 / kCrtWN;
 
-using CrtAcc = intx16[4][4];
+using CrtAcc = intx16[kCrtFM][kCrtFN];
 
 __device__ __forceinline__ void crt_mainloop(const int8_t *__restrict__ Af,
                                              const int8_t *__restrict__ Bf,
                                              int Kp, int m0, int n0,
                                              char *lds, CrtAcc &acc) {
   constexpr int NTH = 256, PT = kCrtBM * (kCrtBK / 16) / NTH;  // 8
+  constexpr int FM = kCrtFM, FN = kCrtFN;
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
-  const int wm = wave >> 1, wn = wave & 1;
+  const int wm = wave / kCrtWN, wn = wave % kCrtWN;
   const int lrow = lane & 31, lgrp = lane >> 5;
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < FM; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = intx16{};
+    for (int j = 0; j < FN; ++j) acc[i][j] = intx16{};
   // A loader: a K step of the tile is 8 row blocks x 4 chunks of 1 KiB
   // fragments; load i of wave w is fragment (row block i, chunk w), lane l
   // holding row 32 i + l % 32, bytes 32 w + 16 (l / 32) of the step.
@@ -935,32 +945,32 @@ __device__ __forceinline__ void crt_mainloop(const int8_t *__restrict__ Af,
       *reinterpret_cast<vec4u *>(b + soff + 32 * i * kCrtRowB) = st[i];
   };
   auto buf = [&](int kt) { return lds + (kt & 1) * kCrtBuf; };
-  const int aoff = (wm * 128 + lrow) * kCrtRowB + lgrp * 16;
-  vec4u fa[2][4];
-  auto frags = [&](vec4u(&a)[4], const char *cur, int c) {
+  const int aoff = (wm * FM * 32 + lrow) * kCrtRowB + lgrp * 16;
+  vec4u fa[2][FM];
+  auto frags = [&](vec4u(&a)[FM], const char *cur, int c) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < FM; ++i)
       a[i] = *reinterpret_cast<const vec4u *>(cur + aoff + i * 32 * kCrtRowB +
                                               c * 32);
   };
-  // B fragment (block j, chunk q) at bbase + j * jstride + q * 1024. Two
-  // fragment sets: step kt uses set kt % 2 and refills each chunk's
-  // fragments with step kt + 2's right after its MFMAs.
+  // B fragment (block j, chunk q) at bbase + j * jstride + q * 1024. The
+  // fragments of chunk c of step kt + 1 replace those of step kt right
+  // after the chunk's MFMAs.
   const int64_t jstride = (int64_t)Kp * 32;
   const int8_t *bbase =
-      Bf + (int64_t)((n0 + wn * 128) >> 5) * jstride + lane * 16;
-  vec4u fbX[4][4];
-  auto bload = [&](vec4u(&fb)[4][4], int kt, int c) {
+      Bf + (int64_t)((n0 + wn * FN * 32) >> 5) * jstride + lane * 16;
+  vec4u fbX[4][FN];
+  auto bload = [&](vec4u(&fb)[4][FN], int kt, int c) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < FN; ++j)
       fb[c][j] = *reinterpret_cast<const vec4u *>(
           bbase + j * jstride + (int64_t)(4 * kt + c) * 1024);
   };
-  auto mfmas = [&](const vec4u(&a)[4], const vec4u(&b)[4]) {
+  auto mfmas = [&](const vec4u(&a)[FM], const vec4u(&b)[FN]) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < FM; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
+      for (int j = 0; j < FN; ++j)
         acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(
             bitcast<intx4>(a[i]), bitcast<intx4>(b[j]), acc[i][j], 0, 0, 0);
   };
@@ -990,27 +1000,29 @@ __device__ __forceinline__ void crt_mainloop(const int8_t *__restrict__ Af,
   __builtin_amdgcn_sched_barrier(0);
   __syncthreads();
   frags(fa[0], buf(0), 0);
-  // Issue order inside a chunk: one MFMA, then one of the chunk's memory
-  // instructions, so the MFMA pipe stays busy while they are sent.
-  // The closing sched_barrier keeps each chunk's instructions in its own
-  // region, so loads are issued in the order they are needed.
+  // Issue order inside a chunk of FM x FN = 16 MFMAs: the LDS reads after
+  // the first MFMAs, the LDS writes and global loads after the last ones,
+  // so the MFMA pipe stays busy while they are sent. The closing
+  // sched_barrier keeps each chunk's instructions in its own region, so
+  // loads are issued in the order they are needed.
   auto interleave = [](int reads, int writes, int vmem) {
-    for (int i = 0; i < 16; ++i) {
+    for (int i = 0; i < FM * FN; ++i) {
       __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
       if (i < reads) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
-      if (i >= 16 - writes)
+      if (i >= FM * FN - writes)
         __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
-      if (i >= 16 - vmem) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+      if (i >= FM * FN - vmem)
+        __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
     }
     __builtin_amdgcn_sched_barrier(0);
   };
-  // Step kt with B fragments fb (set kt % 2) and A staging set sn (holding
-  // tile kt + 1, refilled with tile kt + 3). Every step runs this schedule:
-  // loads of steps past the last one reload the last step's data, which is
-  // never used (the tile they stage goes to the LDS buffer no later step
-  // reads), so no condition splits a chunk's scheduling region.
+  // Step kt with B fragments fb and A staging set sn (holding tile kt + 1,
+  // refilled with tile kt + 3). Every step runs this schedule: loads of
+  // steps past the last one reload the last step's data, which is never
+  // used (the tile they stage goes to the LDS buffer no later step reads),
+  // so no condition splits a chunk's scheduling region.
   const int last = nk - 1;
-  auto steady = [&](int kt, vec4u(&fb)[4][4], vec4u(&sn)[PT]) {
+  auto steady = [&](int kt, vec4u(&fb)[4][FN], vec4u(&sn)[PT]) {
     const char *cur = buf(kt);
     char *nxt = buf(kt + 1);
     const int kb = kt + 1 < last ? kt + 1 : last;
@@ -1019,21 +1031,21 @@ __device__ __forceinline__ void crt_mainloop(const int8_t *__restrict__ Af,
     store_a(sn, nxt);
     mfmas(fa[0], fb[0]);
     bload(fb, kb, 0);
-    interleave(4, 8, 4);
+    interleave(FM, PT, FN);
     frags(fa[0], cur, 2);
     gload(sn, ka * kCrtBK);
     mfmas(fa[1], fb[1]);
     bload(fb, kb, 1);
-    interleave(4, 0, 12);
+    interleave(FM, 0, PT + FN);
     frags(fa[1], cur, 3);
     mfmas(fa[0], fb[2]);
     bload(fb, kb, 2);
-    interleave(4, 0, 4);
+    interleave(FM, 0, FN);
     __syncthreads();
     frags(fa[0], nxt, 0);
     mfmas(fa[1], fb[3]);
     bload(fb, kb, 3);
-    interleave(4, 0, 4);
+    interleave(FM, 0, FN);
   };
   int kt = 0;
   do {  // nk is even and positive
@@ -1053,7 +1065,7 @@ __global__ __launch_bounds__(256) void crt_gemm4_kernel(CrtParams p) {
   CrtAcc acc;
   crt_mainloop(p.Ap + t * p.astride, p.Bp + t * p.bstride, (int)p.Kp,
                tm * kCrtBM, tn * kCrtBM, lds, acc);
-  crt_store_tile<kCrtBM, kCrtBM, 2, 2, kCrtLds>(
+  crt_store_tile<kCrtBM, kCrtBM, kCrtWM, kCrtWN, kCrtLds>(
       acc, mod, p.R + (int64_t)t * p.M * p.N, p.M, p.N, tm * kCrtBM,
       tn * kCrtBM, lds);
 }
