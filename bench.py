@@ -399,6 +399,24 @@ def gemm_section(hc, h, torch, oracle, n, reps=20, nc=4096):
                                   "latency_ms": round(lat, 4),
                                   "peak_frac": round(2.0 * m ** 3 / (ms * 1e-3) / 1e12
                                                      / PEAK_TFLOPS["fp32"], 4)}
+    # For scale: torch's FP32 matmul (hipBLASLt, TF32 off) on the same
+    # row-major operands, timed the same way (not part of the product).
+    torch.backends.cuda.matmul.allow_tf32 = False
+    Am, Bm = A.view(m, m), B.view(m, m)
+    torch.mm(Am, Bm)
+    torch.cuda.synchronize()
+    runs = []
+    for _ in range(5):
+        s.record()
+        for _ in range(50):
+            torch.mm(Am, Bm)
+        e.record()
+        torch.cuda.synchronize()
+        runs.append(s.elapsed_time(e) / 50)
+    tms = sorted(runs)[len(runs) // 2]
+    out["sgemm_rowmajor_1024"]["torch_matmul_ms"] = round(tms, 4)
+    out["sgemm_rowmajor_1024"]["torch_matmul_peak_frac"] = round(
+        2.0 * m ** 3 / (tms * 1e-3) / 1e12 / PEAK_TFLOPS["fp32"], 4)
     return out
 
 
