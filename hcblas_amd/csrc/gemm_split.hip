@@ -86,7 +86,7 @@ struct TileStage {
 template <typename T, int RT, int KT>
 struct TileRK {
   T v[RT][KT + 1];
-  template <bool KC, bool NTC>
+  template <bool KC>
   __device__ __forceinline__ void load_as(const T *__restrict__ X, int64_t ld,
                                           bool vec, int R, int K, int r0,
                                           int k0) {
@@ -107,13 +107,9 @@ struct TileRK {
       T t[VE];
       const bool full = KC ? (r < R && k + VE <= K) : (k < K && r + VE <= R);
       if (vec && full) {
-        // The operand is read once: non-temporal loads, except for columns
-        // along r when NTC is false (measured slower for the residue
-        // kernel's 32-element pieces).
+        // The operand is read once: non-temporal loads.
         const vec4u w =
-            (KC || NTC)
-                ? __builtin_nontemporal_load(reinterpret_cast<const vec4u *>(src))
-                : *reinterpret_cast<const vec4u *>(src);
+            __builtin_nontemporal_load(reinterpret_cast<const vec4u *>(src));
         __builtin_memcpy(t, &w, 16);
       } else {
 #pragma unroll
@@ -131,14 +127,13 @@ struct TileRK {
       }
     }
   }
-  template <bool NTC = true>
   __device__ __forceinline__ void load(const T *__restrict__ X, int64_t ld,
                                        bool kcont, bool vec, int R, int K,
                                        int r0, int k0) {
     if (kcont)
-      load_as<true, NTC>(X, ld, vec, R, K, r0, k0);
+      load_as<true>(X, ld, vec, R, K, r0, k0);
     else
-      load_as<false, NTC>(X, ld, vec, R, K, r0, k0);
+      load_as<false>(X, ld, vec, R, K, r0, k0);
     __syncthreads();
   }
 };
@@ -749,24 +744,29 @@ __device__ __forceinline__ void write_residues16(const ResidueIn<WIDE> &in,
 // (r, 16-k group g) at lane (r % 32) + 32 * (g % 2) -- the operand layout of
 // v_mfma_i32_32x32x32_i8, so a wave loads a fragment with one coalesced
 // 16-byte load per lane. Rows from R to Rpad are not written. A workgroup
-// stages a 32 x 128 tile through LDS, reading along the source's contiguous
-// dimension; each thread then converts 16 consecutive k of one row and
-// writes one 16-byte vector per plane, so a wave writes two whole 1 KiB
-// runs of each plane.
+// stages a tile through LDS, reading along the source's contiguous
+// dimension: 32 rows x 128 k for a k-contiguous operand (KC, 512-byte row
+// pieces), 128 rows x 32 k for an r-contiguous one (512-byte column pieces;
+// a 32 x 128 tile read them in 128-byte pieces, 25% slower at 8192^2).
+// Each thread then converts 16 consecutive k of one row and writes one
+// 16-byte vector per plane, so a wave writes whole 1 KiB runs of each
+// plane.
 // FP32 input: bits - e_r keeps |v| <= 2^30, and the scaling and rounding are
 // exact in float (values below 2^-126 round to 0 either way); FP64 input:
 // |v| <= 2^62, split exactly into 32-bit halves in double.
-template <typename T, int NMOD>
+template <typename T, int NMOD, bool KC>
 __global__ __launch_bounds__(256) void crt_residue_kernel(
-    int R, int K, const T *__restrict__ X, int64_t ld, bool kcont, bool vec,
+    int R, int K, const T *__restrict__ X, int64_t ld, bool vec,
     const int *__restrict__ exps, int bits, int8_t *__restrict__ P,
     int64_t Kp, int64_t pstride) {
   constexpr bool WIDE = sizeof(T) == 8;
-  __shared__ TileRK<T, 32, 128> st;
-  const int r0 = blockIdx.y * 32, k0 = blockIdx.x * 128;
+  constexpr int RT = KC ? 32 : 128, KT = KC ? 128 : 32;
+  __shared__ TileRK<T, RT, KT> st;
+  const int r0 = blockIdx.y * RT, k0 = blockIdx.x * KT;
   const int tid = threadIdx.x;
-  st.template load<false>(X, ld, kcont, vec, R, K, r0, k0);
-  const int row = tid >> 3, kq = (tid & 7) * 16;
+  st.template load_as<KC>(X, ld, vec, R, K, r0, k0);
+  __syncthreads();
+  const int row = tid / (KT / 16), kq = (tid % (KT / 16)) * 16;
   const int r = r0 + row;
   if (r >= R) return;
   const int e = exps[r];
@@ -1274,9 +1274,14 @@ void launch_crt_residues(hipStream_t s, int R, int K, const T *X, int64_t ld,
                          bool kcont, const int *exps, int bits, int8_t *P,
                          int64_t Kp, int64_t pstride) {
   // Kp is a multiple of 128 (the INT8 GEMM's K step).
-  dim3 g((unsigned)(Kp / 128), (unsigned)cdiv(R, 32));
-  crt_residue_kernel<T, NMOD><<<g, 256, 0, s>>>(
-      R, K, X, ld, kcont, vec_aligned(X, ld), exps, bits, P, Kp, pstride);
+  if (kcont)
+    crt_residue_kernel<T, NMOD, true>
+        <<<dim3((unsigned)(Kp / 128), (unsigned)cdiv(R, 32)), 256, 0, s>>>(
+            R, K, X, ld, vec_aligned(X, ld), exps, bits, P, Kp, pstride);
+  else
+    crt_residue_kernel<T, NMOD, false>
+        <<<dim3((unsigned)(Kp / 32), (unsigned)cdiv(R, 128)), 256, 0, s>>>(
+            R, K, X, ld, vec_aligned(X, ld), exps, bits, P, Kp, pstride);
 }
 template <int NMOD, typename OutT>
 void launch_crt_combine(hipStream_t s, int M, int N, const uint8_t *R,
