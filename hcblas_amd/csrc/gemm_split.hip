@@ -801,32 +801,26 @@ struct CrtParams {
   const int8_t *Ap, *Bp;  // residue planes in fragment order (plane t at
                           // Ap + t * astride, Bp + t * bstride)
   int64_t astride, bstride;
-  uint8_t *R;             // R[t][n][m]: (A'B')(m, n) mod m_t in [0, m_t)
+  uint8_t *R;  // residues of A'B' mod m_t in [0, m_t), in 32 x 32 blocks
+               // (crt_store_blocks); plane t at R + t * Mb * Nb * 1024
+  int Mb, Nb;  // blocks along m and n
   int tiles_m, tiles_n;
 };
 
-// Residues of one BM x BN tile of A'B' (rows along m, columns along n)
-// modulo `mod`, written to R (n-major, m contiguous) through LDS. In the
-// 32x32 MFMA output layout a lane's values 4g..4g+3 of a block are four
-// consecutive rows (m), so they pack into one dword of the LDS tile
-// [n][m] (row stride BM + 16 bytes); the tile then leaves with 16-byte
-// stores, a row of BM bytes per 16 lanes. Storing every value as its own
-// byte straight from the accumulator layout costs one global store
-// instruction per value (128 per lane for 256 x 256 tiles).
+// Residues modulo `mod` of the 32 x 32 blocks a wave holds, written in the
+// accumulator's own layout: block (bm, bn) of a plane is the 1 KiB run at
+// (bn * Mb + bm) * 1024, lane l's 16 bytes being rows 8 (r / 4) + 4 (l / 32)
+// + r % 4 (r = 0..15) of column l % 32 of the block -- the output layout of
+// v_mfma_i32_32x32x32_i8, so each block leaves as one coalesced 16-byte
+// store per lane, with no LDS pass and no barrier. Blocks past the matrix
+// (bm >= Mb or bn >= Nb) are not written.
 // a mod m: |a| <= K * 128^2 <= 2^30, so q = floor(float(a) / m) is within
 // one of the true quotient and |q| < 2^23 (24-bit multiply).
-template <int BM, int BN, int WM, int WN, int LDS_BYTES, typename AccArr>
-__device__ __forceinline__ void crt_store_tile(const AccArr &acc, int mod,
-                                               uint8_t *R, int M, int N,
-                                               int m0, int n0, char *lds) {
-  constexpr int MI = 32;
-  using WT = WaveTile<Elem::I8, BM, BN, WM, WN, MI>;
-  constexpr int LROW = BM + 16;
-  static_assert(BN * LROW <= LDS_BYTES,
-                "residue tile fits the LDS of the main loop");
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int wm = wave / WN, wn = wave % WN;
-  const int lrow = lane & 31, lgrp = lane >> 5;
+template <int FM, int FN, typename AccArr>
+__device__ __forceinline__ void crt_store_blocks(const AccArr &acc, int mod,
+                                                 uint8_t *Rt, int Mb, int Nb,
+                                                 int bm0, int bn0) {
+  const int lane = threadIdx.x & 63;
   const float inv = 1.0f / (float)mod;
   auto red = [&](int a) {
     const int q = (int)__builtin_floorf((float)a * inv);
@@ -835,39 +829,23 @@ __device__ __forceinline__ void crt_store_tile(const AccArr &acc, int mod,
     u -= u >= mod ? mod : 0;
     return u;
   };
-  __syncthreads();
 #pragma unroll
-  for (int i = 0; i < WT::FM; ++i)
+  for (int i = 0; i < FM; ++i)
 #pragma unroll
-    for (int j = 0; j < WT::FN; ++j) {
-      const int n = wn * WT::WTN + j * MI + lrow;
+    for (int j = 0; j < FN; ++j) {
+      const int bm = bm0 + i, bn = bn0 + j;
+      if (bm >= Mb || bn >= Nb) continue;
+      vec4u w;
+      uint32_t *wd = reinterpret_cast<uint32_t *>(&w);
 #pragma unroll
-      for (int g = 0; g < 4; ++g) {
-        const int m = wm * WT::WTM + i * MI + 8 * g + 4 * lgrp;
-        const uint32_t w =
-            pack_bytes(red(acc[i][j][4 * g]), red(acc[i][j][4 * g + 1]),
-                       red(acc[i][j][4 * g + 2]), red(acc[i][j][4 * g + 3]));
-        *reinterpret_cast<uint32_t *>(lds + n * LROW + m) = w;
-      }
+      for (int g = 0; g < 4; ++g)
+        wd[g] = pack_bytes(red(acc[i][j][4 * g]), red(acc[i][j][4 * g + 1]),
+                           red(acc[i][j][4 * g + 2]),
+                           red(acc[i][j][4 * g + 3]));
+      __builtin_nontemporal_store(
+          w, reinterpret_cast<vec4u *>(Rt + ((int64_t)bn * Mb + bm) * 1024 +
+                                       lane * 16));
     }
-  __syncthreads();
-  constexpr int CH = BM / 16;  // 16-byte chunks per row
-  constexpr int NTH = 64 * WM * WN;
-  const bool vec = (M & 15) == 0;
-#pragma unroll
-  for (int c = threadIdx.x; c < BN * CH; c += NTH) {
-    const int nl = c / CH, ml = (c % CH) * 16;
-    const int n = n0 + nl, m = m0 + ml;
-    if (n >= N || m >= M) continue;
-    uint8_t *dst = R + (int64_t)n * M + m;
-    const char *src = lds + nl * LROW + ml;
-    if (vec) {
-      *reinterpret_cast<vec4u *>(dst) = *reinterpret_cast<const vec4u *>(src);
-    } else {
-      const int e = M - m < 16 ? M - m : 16;
-      for (int b = 0; b < e; ++b) dst[b] = (uint8_t)src[b];
-    }
-  }
 }
 
 // Main loop of the residue GEMM: a 256 x 256 tile of one modulus' product
@@ -1065,9 +1043,11 @@ __global__ __launch_bounds__(256) void crt_gemm4_kernel(CrtParams p) {
   CrtAcc acc;
   crt_mainloop(p.Ap + t * p.astride, p.Bp + t * p.bstride, (int)p.Kp,
                tm * kCrtBM, tn * kCrtBM, lds, acc);
-  crt_store_tile<kCrtBM, kCrtBM, kCrtWM, kCrtWN, kCrtLds>(
-      acc, mod, p.R + (int64_t)t * p.M * p.N, p.M, p.N, tm * kCrtBM,
-      tn * kCrtBM, lds);
+  const int wave = threadIdx.x >> 6;
+  crt_store_blocks<kCrtFM, kCrtFN>(
+      acc, mod, p.R + (int64_t)t * p.Mb * p.Nb * 1024, p.Mb, p.Nb,
+      (tm * kCrtBM >> 5) + (wave / kCrtWN) * kCrtFM,
+      (tn * kCrtBM >> 5) + (wave % kCrtWN) * kCrtFN);
 }
 
 // Reconstruction by the explicit CRT. With P = prod m_t, M_t = P / m_t and
@@ -1216,57 +1196,138 @@ __device__ __forceinline__ OutT crt_out(double v, double beta, OutT c) {
   }
 }
 
-// C(m, n) = alpha * 2^(e_m + f_n - shift) * CRT(R[.][n][m]) + beta * C(m, n).
-// W = 4 (M % 4 == 0): a thread takes 4 consecutive m of one column, reads one
-// dword per residue plane and, when C is vector aligned (vec_c), writes its
-// outputs with 16-byte stores. W = 1: one element. (Sixteen elements per
-// thread measured slower: the serial reconstructions and the register use
-// cost more parallelism than the wider loads gain.)
-template <int NMOD, typename OutT, int W>
-__global__ __launch_bounds__(256) void crt_combine_kernel(
-    int M, int N, const uint8_t *__restrict__ R, const int *__restrict__ ea,
-    const int *__restrict__ eb, int shift, double alpha, double beta,
-    OutT *C, int64_t ldc, bool vec_c) {
-  const int64_t MN = (int64_t)M * N;
-  const int64_t idx = ((int64_t)blockIdx.x * 256 + threadIdx.x) * W;
-  if (idx >= MN) return;
-  const int m = (int)(idx % M), n = (int)(idx / M);
-  const int ce = eb[n];
-  OutT *dst = C + (int64_t)n * ldc + m;
-  if constexpr (W == 4) {
-    uint32_t w[NMOD];
-#pragma unroll
-    for (int t = 0; t < NMOD; ++t)
-      w[t] = *reinterpret_cast<const uint32_t *>(R + t * MN + idx);
-    OutT o[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      int u[NMOD];
-#pragma unroll
-      for (int t = 0; t < NMOD; ++t) u[t] = (w[t] >> (8 * j)) & 0xff;
-      const double v = crt_scaled(crt_value<NMOD>(u), ea[m + j], ce, shift,
-                                  alpha);
-      o[j] = crt_out<OutT>(v, beta, beta != 0.0 ? dst[j] : OutT(0));
-    }
-    if (vec_c) {
-      constexpr int VE = 16 / (int)sizeof(OutT);
-#pragma unroll
-      for (int q = 0; q < 4 / VE; ++q) {
-        vec4u cv;
-        __builtin_memcpy(&cv, o + q * VE, 16);
-        __builtin_nontemporal_store(cv, reinterpret_cast<vec4u *>(dst + q * VE));
-      }
-    } else {
-#pragma unroll
-      for (int j = 0; j < 4; ++j) dst[j] = o[j];
-    }
+// C(m, n) = alpha * 2^(e_m + f_n - shift) * CRT(u(m, n)) + beta * C(m, n),
+// the residues u read from the 32 x 32 blocks crt_store_blocks writes.
+// Every block's outputs are staged through an LDS tile [column][row] (row
+// stride 32 + VE elements, VE = elements per 16 bytes) and leave with
+// 16-byte stores covering whole 128-byte runs of columns. (Written straight
+// from the accumulator layout, each store touched 32 columns with 16-byte
+// pieces: 2.3x slower.)
+//   - Up to 12 moduli (crt_combine_waves): one wave per block; lane l loads
+//     its 16 bytes of the block from every plane (one coalesced 16-byte
+//     load per plane) and reconstructs rows 8 q + 4 (l / 32) + e (q, e =
+//     0..3) of column l % 32. 8 moduli at 8192^2: 150-158 us, against
+//     171-184 for the former [n][m] layout's four rows per thread.
+//   - More moduli (crt_combine_groups): one workgroup per block, thread t
+//     taking dword t % 4 of lane t / 4's 16 bytes (each plane one contiguous
+//     1 KiB run), which keeps the registers of the 16-modulus FP64 form low
+//     (whole 16-byte slots per lane cost it occupancy: 443-465 us against
+//     420-485 this way).
+template <int NMOD, typename OutT>
+__device__ __forceinline__ void crt_combine_rows(
+    const uint32_t *w, int m, int n, int M, const int *__restrict__ ea, int ce,
+    int shift, double alpha, double beta, const OutT *C, int64_t ldc,
+    OutT *trow) {
+  constexpr int VE = 16 / (int)sizeof(OutT);
+  int er[4];
+  if (m + 4 <= M) {
+    const int4 e4 = *reinterpret_cast<const int4 *>(ea + m);
+    er[0] = e4.x, er[1] = e4.y, er[2] = e4.z, er[3] = e4.w;
   } else {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) er[e] = m + e < M ? ea[m + e] : kNoScale;
+  }
+  OutT o[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
     int u[NMOD];
 #pragma unroll
-    for (int t = 0; t < NMOD; ++t) u[t] = R[t * MN + idx];
-    const double v = crt_scaled(crt_value<NMOD>(u), ea[m], ce, shift, alpha);
-    *dst = crt_out<OutT>(v, beta, beta != 0.0 ? *dst : OutT(0));
+    for (int i = 0; i < NMOD; ++i) u[i] = (w[i] >> (8 * e)) & 0xff;
+    const double v = crt_scaled(crt_value<NMOD>(u), er[e], ce, shift, alpha);
+    o[e] = crt_out<OutT>(
+        v, beta, beta != 0.0 && m + e < M ? C[(int64_t)n * ldc + m + e] : OutT(0));
   }
+#pragma unroll
+  for (int h = 0; h < 4 / VE; ++h) {
+    vec4u cv;
+    __builtin_memcpy(&cv, o + h * VE, 16);
+    *reinterpret_cast<vec4u *>(trow + h * VE) = cv;
+  }
+}
+// The LDS tile of block (bm, bn) to C, threads t0, t0 + step, ... .
+template <typename OutT>
+__device__ __forceinline__ void crt_tile_out(const OutT *tile, int bm, int bn,
+                                             int M, int N, OutT *C,
+                                             int64_t ldc, bool vec_c, int t0,
+                                             int step) {
+  constexpr int VE = 16 / (int)sizeof(OutT), LS = 32 + VE;
+  constexpr int LPC = 32 / VE;  // threads per column run
+  for (int v = t0; v < 32 * LPC; v += step) {
+    const int cc = v / LPC, rv = (v % LPC) * VE;
+    const int nn = bn * 32 + cc, mm = bm * 32 + rv;
+    if (nn >= N || mm >= M) continue;
+    OutT *dst = C + (int64_t)nn * ldc + mm;
+    const OutT *src = tile + cc * LS + rv;
+    if (vec_c && mm + VE <= M) {
+      __builtin_nontemporal_store(*reinterpret_cast<const vec4u *>(src),
+                                  reinterpret_cast<vec4u *>(dst));
+    } else {
+      for (int e = 0; e < VE && mm + e < M; ++e) dst[e] = src[e];
+    }
+  }
+}
+template <int NMOD, typename OutT>
+__global__ __launch_bounds__(256) void crt_combine_waves(
+    int M, int N, int Mb, int nblk, const uint8_t *__restrict__ R,
+    const int *__restrict__ ea, const int *__restrict__ eb, int shift,
+    double alpha, double beta, OutT *C, int64_t ldc, bool vec_c) {
+  constexpr int LS = 32 + 16 / (int)sizeof(OutT);
+  __shared__ __attribute__((aligned(16))) OutT tile[4][32 * LS];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int blk = blockIdx.x * 4 + wave;
+  if (blk >= nblk) return;  // wave-uniform; no barrier follows
+  const int bm = blk % Mb, bn = blk / Mb;
+  const int c = lane & 31, n = bn * 32 + c;
+  const int64_t plane = (int64_t)nblk * 1024;
+  vec4u w[NMOD];
+#pragma unroll
+  for (int i = 0; i < NMOD; ++i)
+    w[i] = __builtin_nontemporal_load(reinterpret_cast<const vec4u *>(
+        R + i * plane + (int64_t)blk * 1024 + lane * 16));
+  OutT *tw = tile[wave];
+  if (n < N) {
+    const int ce = eb[n];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int r0 = 8 * q + 4 * (lane >> 5), m = bm * 32 + r0;
+      if (m >= M) break;
+      uint32_t wq[NMOD];
+#pragma unroll
+      for (int i = 0; i < NMOD; ++i)
+        wq[i] = reinterpret_cast<const uint32_t *>(&w[i])[q];
+      crt_combine_rows<NMOD, OutT>(wq, m, n, M, ea, ce, shift, alpha, beta, C,
+                                   ldc, tw + c * LS + r0);
+    }
+  }
+  // Within the wave only: the LDS writes above and the reads below.
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  crt_tile_out<OutT>(tw, bm, bn, M, N, C, ldc, vec_c, lane, 64);
+}
+template <int NMOD, typename OutT>
+__global__ __launch_bounds__(256) void crt_combine_groups(
+    int M, int N, int Mb, int nblk, const uint8_t *__restrict__ R,
+    const int *__restrict__ ea, const int *__restrict__ eb, int shift,
+    double alpha, double beta, OutT *C, int64_t ldc, bool vec_c) {
+  constexpr int LS = 32 + 16 / (int)sizeof(OutT);
+  __shared__ __attribute__((aligned(16))) OutT tile[32 * LS];
+  const int blk = blockIdx.x;
+  const int bm = blk % Mb, bn = blk / Mb;
+  const int t = threadIdx.x, q = t & 3, lr = t >> 2;
+  const int c = lr & 31, r0 = 8 * q + 4 * (lr >> 5);
+  const int n = bn * 32 + c, m = bm * 32 + r0;
+  const int64_t plane = (int64_t)nblk * 1024;
+  uint32_t w[NMOD];
+#pragma unroll
+  for (int i = 0; i < NMOD; ++i)
+    w[i] = __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(
+        R + i * plane + (int64_t)blk * 1024 + t * 4));
+  if (n < N && m < M)
+    crt_combine_rows<NMOD, OutT>(w, m, n, M, ea, eb[n], shift, alpha, beta, C,
+                                 ldc, tile + c * LS + r0);
+  __syncthreads();
+  crt_tile_out<OutT>(tile, bm, bn, M, N, C, ldc, vec_c, t, 256);
 }
 
 template <int NMOD, typename T>
@@ -1287,14 +1348,15 @@ template <int NMOD, typename OutT>
 void launch_crt_combine(hipStream_t s, int M, int N, const uint8_t *R,
                         const int *ea, const int *eb, int shift, double alpha,
                         double beta, OutT *C, int64_t ldc) {
-  const int64_t MN = (int64_t)M * N;
-  if (M % 4 == 0)
-    crt_combine_kernel<NMOD, OutT, 4>
-        <<<(unsigned)cdiv(MN / 4, 256), 256, 0, s>>>(
-            M, N, R, ea, eb, shift, alpha, beta, C, ldc, vec_aligned(C, ldc));
+  const int Mb = (int)cdiv(M, 32), nblk = Mb * (int)cdiv(N, 32);
+  if constexpr (NMOD <= 12)
+    crt_combine_waves<NMOD, OutT><<<(unsigned)cdiv(nblk, 4), 256, 0, s>>>(
+        M, N, Mb, nblk, R, ea, eb, shift, alpha, beta, C, ldc,
+        vec_aligned(C, ldc));
   else
-    crt_combine_kernel<NMOD, OutT, 1><<<(unsigned)cdiv(MN, 256), 256, 0, s>>>(
-        M, N, R, ea, eb, shift, alpha, beta, C, ldc, false);
+    crt_combine_groups<NMOD, OutT><<<(unsigned)nblk, 256, 0, s>>>(
+        M, N, Mb, nblk, R, ea, eb, shift, alpha, beta, C, ldc,
+        vec_aligned(C, ldc));
 }
 
 // Runtime modulus count -> template instance (6..20).
@@ -1514,7 +1576,8 @@ hcblasStatus_t gemm_ozaki_crt(hcblasHandle_t h, const Canon &c, double alpha,
   const int64_t Npad = cdiv(c.N, kCrtBM) * kCrtBM;
   const size_t a_bytes = (size_t)nmax * Mpad * Kp;
   const size_t b_bytes = (size_t)nmax * Npad * Kp;
-  const size_t r_bytes = (size_t)nmax * c.M * c.N;
+  const int Mb = (int)cdiv(c.M, 32), Nb = (int)cdiv(c.N, 32);
+  const size_t r_bytes = (size_t)nmax * Mb * Nb * 1024;
   const size_t e_bytes = sizeof(int) * ((size_t)c.M + c.N);
   auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
   if (ensure_workspace(h->split_ws, up(a_bytes) + up(b_bytes) + up(r_bytes) +
@@ -1565,6 +1628,8 @@ hcblasStatus_t gemm_ozaki_crt(hcblasHandle_t h, const Canon &c, double alpha,
       p.astride = Mpad * Kp;
       p.bstride = Npad * Kp;
       p.R = R;
+      p.Mb = Mb;
+      p.Nb = Nb;
       p.tiles_m = (int)cdiv(c.M, kCrtBM);
       p.tiles_n = (int)cdiv(c.N, kCrtBM);
       crt_gemm4_kernel<<<dim3((unsigned)(p.tiles_m * p.tiles_n), 1,
@@ -1623,7 +1688,8 @@ hcblasStatus_t gemm_ozaki_crt_f32(hcblasHandle_t h, const Canon &c,
   const int64_t Npad = cdiv(c.N, kCrtBM) * kCrtBM;
   const size_t a_bytes = (size_t)nmod * Mpad * Kp;
   const size_t b_bytes = (size_t)nmod * Npad * Kp;
-  const size_t r_bytes = (size_t)nmod * c.M * c.N;
+  const int Mb = (int)cdiv(c.M, 32), Nb = (int)cdiv(c.N, 32);
+  const size_t r_bytes = (size_t)nmod * Mb * Nb * 1024;
   const size_t e_bytes = sizeof(int) * ((size_t)c.M + c.N);
   const size_t p_bytes = sizeof(double) * ((size_t)pa * c.M + (size_t)pb * c.N);
   auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
@@ -1682,6 +1748,8 @@ hcblasStatus_t gemm_ozaki_crt_f32(hcblasHandle_t h, const Canon &c,
       p.astride = Mpad * Kp;
       p.bstride = Npad * Kp;
       p.R = R;
+      p.Mb = Mb;
+      p.Nb = Nb;
       p.tiles_m = (int)cdiv(c.M, kCrtBM);
       p.tiles_n = (int)cdiv(c.N, kCrtBM);
       crt_gemm4_kernel<<<dim3((unsigned)(p.tiles_m * p.tiles_n), 1,
