@@ -912,15 +912,19 @@ __device__ __forceinline__ void crt_mainloop(const int8_t *__restrict__ Af,
   // Two staging sets: tile j of A goes through set j % 2 and is loaded
   // seven chunks before the chunk that stores it.
   vec4u sa[2][PT];
-  auto gload = [&](vec4u(&st)[PT], int k0) {  // k0: byte offset of the step
+  // h: half of the tile's loads / stores (rows 0-127 or 128-255), or -1
+  // for all of them.
+  auto gload = [&](vec4u(&st)[PT], int k0, int h) {  // k0: step's byte offset
 #pragma unroll
     for (int i = 0; i < PT; ++i)
-      st[i] = *reinterpret_cast<const vec4u *>(ga + i * fstep + k0 * 32);
+      if (h < 0 || i / (PT / 2) == h)
+        st[i] = *reinterpret_cast<const vec4u *>(ga + i * fstep + k0 * 32);
   };
-  auto store_a = [&](const vec4u(&st)[PT], char *b) {
+  auto store_a = [&](const vec4u(&st)[PT], char *b, int h) {
 #pragma unroll
     for (int i = 0; i < PT; ++i)
-      *reinterpret_cast<vec4u *>(b + soff + 32 * i * kCrtRowB) = st[i];
+      if (h < 0 || i / (PT / 2) == h)
+        *reinterpret_cast<vec4u *>(b + soff + 32 * i * kCrtRowB) = st[i];
   };
   auto buf = [&](int kt) { return lds + (kt & 1) * kCrtBuf; };
   const int aoff = (wm * FM * 32 + lrow) * kCrtRowB + lgrp * 16;
@@ -955,26 +959,31 @@ __device__ __forceinline__ void crt_mainloop(const int8_t *__restrict__ Af,
   const int nk = Kp / kCrtBK;
   if (nk == 0) return;
   // The prologue issues its loads in the order the loop does (tile 1, B
-  // chunk 0, tile 2, B chunks 1-3), unconditionally (tiles past the planes'
-  // length reload tile 0 and are never stored), so at the loop head every
-  // register has as many younger loads outstanding on entry as on the back
-  // edge. The wait counts the compiler derives for the loop from the merge
-  // of the two paths are then the loop's own; a skipped load on either path
-  // would make it wait for loads issued a chunk earlier.
-  gload(sa[0], 0);
+  // chunk 0, first half of tile 2, B chunk 1, second half, B chunks 2-3),
+  // unconditionally (tiles past the planes' length reload tile 0 and are
+  // never stored), so at the loop head every register has at least as many
+  // younger loads outstanding on entry as on the back edge. The wait counts
+  // the compiler derives for the loop from the merge of the two paths are
+  // then the loop's own; a skipped load on either path would make it wait
+  // for loads issued a chunk earlier.
+  const int k2 = nk > 2 ? 2 * kCrtBK : 0;
+  gload(sa[0], 0, -1);
   __builtin_amdgcn_sched_barrier(0);
-  gload(sa[1], nk > 1 ? kCrtBK : 0);
+  gload(sa[1], nk > 1 ? kCrtBK : 0, -1);
   __builtin_amdgcn_sched_barrier(0);
   bload(fbX, 0, 0);
   __builtin_amdgcn_sched_barrier(0);
-  store_a(sa[0], buf(0));
+  store_a(sa[0], buf(0), -1);
   __builtin_amdgcn_sched_barrier(0);
-  gload(sa[0], nk > 2 ? 2 * kCrtBK : 0);
-#pragma unroll
-  for (int c = 1; c < 4; ++c) {
-    __builtin_amdgcn_sched_barrier(0);
-    bload(fbX, 0, c);
-  }
+  gload(sa[0], k2, 0);
+  __builtin_amdgcn_sched_barrier(0);
+  bload(fbX, 0, 1);
+  __builtin_amdgcn_sched_barrier(0);
+  gload(sa[0], k2, 1);
+  __builtin_amdgcn_sched_barrier(0);
+  bload(fbX, 0, 2);
+  __builtin_amdgcn_sched_barrier(0);
+  bload(fbX, 0, 3);
   __builtin_amdgcn_sched_barrier(0);
   __syncthreads();
   frags(fa[0], buf(0), 0);
@@ -1005,20 +1014,25 @@ __device__ __forceinline__ void crt_mainloop(const int8_t *__restrict__ Af,
     char *nxt = buf(kt + 1);
     const int kb = kt + 1 < last ? kt + 1 : last;
     const int ka = kt + 3 < last ? kt + 3 : last;
+    // The next tile's LDS writes and its set's refill are spread over
+    // chunks 0-2, half a tile at a time, so no chunk issues more than 12
+    // LDS instructions.
     frags(fa[1], cur, 1);
-    store_a(sn, nxt);
+    store_a(sn, nxt, 0);
     mfmas(fa[0], fb[0]);
     bload(fb, kb, 0);
-    interleave(FM, PT, FN);
+    interleave(FM, PT / 2, FN);
     frags(fa[0], cur, 2);
-    gload(sn, ka * kCrtBK);
+    store_a(sn, nxt, 1);
+    gload(sn, ka * kCrtBK, 0);
     mfmas(fa[1], fb[1]);
     bload(fb, kb, 1);
-    interleave(FM, 0, PT + FN);
+    interleave(FM, PT / 2, PT / 2 + FN);
     frags(fa[1], cur, 3);
+    gload(sn, ka * kCrtBK, 1);
     mfmas(fa[0], fb[2]);
     bload(fb, kb, 2);
-    interleave(FM, 0, FN);
+    interleave(FM, 0, PT / 2 + FN);
     __syncthreads();
     frags(fa[0], nxt, 0);
     mfmas(fa[1], fb[3]);
