@@ -436,22 +436,38 @@ __device__ __forceinline__ void gemm_mainloop(
   // Steps with no K tail (KFULL: the caller guarantees K % BK == 0).
   const int nfull = KFULL ? nk : K / BK;
 
-  // MFMAs over one LDS buffer.
+  // MFMAs over one LDS buffer. Small wave tiles (the 64 x 64 tiles of 8
+  // waves, at most 2 fragments of each operand) read chunk c + 1's
+  // fragments into a second register set before chunk c's MFMAs, so the
+  // LDS latency is not paid between chunks; the larger tiles have no
+  // registers to spare for it.
+  constexpr bool PIPE = WM * WN == 8 && FM * FN <= 2;
   auto compute = [&](const char *cur) {
     const char *As = cur;
     const char *Bs = cur + A_BYTES;
-#pragma unroll
-    for (int c = 0; c < BK / CK; ++c) {
-      vec4u af[FM], bf[FN];
+    vec4u afp[PIPE ? 2 : 1][FM], bfp[PIPE ? 2 : 1][FN];
+    auto frag = [&](int c, int set) {
       const int kbyte = (c * CK + lgrp * VEC) * (int)sizeof(T);
 #pragma unroll
       for (int i = 0; i < FM; ++i)
-        af[i] = *reinterpret_cast<const vec4u *>(
+        afp[set][i] = *reinterpret_cast<const vec4u *>(
             As + (wm * WTM + i * MI + lrow) * ROWB + kbyte);
 #pragma unroll
       for (int j = 0; j < FN; ++j)
-        bf[j] = *reinterpret_cast<const vec4u *>(
+        bfp[set][j] = *reinterpret_cast<const vec4u *>(
             Bs + (wn * WTN + j * MI + lrow) * ROWB + kbyte);
+    };
+    if constexpr (PIPE) frag(0, 0);
+#pragma unroll
+    for (int c = 0; c < BK / CK; ++c) {
+      const int set = PIPE ? (c & 1) : 0;
+      if constexpr (PIPE) {
+        if (c + 1 < BK / CK) frag(c + 1, (c + 1) & 1);
+      } else {
+        frag(c, 0);
+      }
+      const vec4u(&af)[FM] = afp[set];
+      const vec4u(&bf)[FN] = bfp[set];
 #pragma unroll
       for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -475,6 +491,16 @@ __device__ __forceinline__ void gemm_mainloop(
 #pragma unroll
           for (int j = 0; j < FN; ++j)
             mfma_step<E, MI, 3>(af[i], bf[j], acc[i][j]);
+      }
+      if constexpr (PIPE) {
+        // The next chunk's LDS reads go out behind the first MFMAs.
+        constexpr int NMF = FM * FN * Tr::STEPS, NRD = FM + FN;
+#pragma unroll
+        for (int i = 0; i < NMF; ++i) {
+          __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+          if (i < NRD) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        }
+        __builtin_amdgcn_sched_barrier(0);
       }
     }
   };
