@@ -192,6 +192,17 @@ struct TileLoader {
   // conflicts for 32 r-vectors of one k).
   static constexpr bool BLOCKED = !KCONT && (ROWS / VEC) % 8 == 0 &&
                                   BK % 8 == 0 && NV % 64 == 0;
+  // FP32 transposing tiles with four vectors per thread: each thread loads
+  // the four k-lines of one 4-row x 4-k block and writes it with four
+  // 16-byte LDS stores along k (a 4 x 4 transpose in its registers) instead
+  // of sixteen 4-byte ones. The 16 lanes of a group take 4 consecutive
+  // r-vectors x 4 consecutive k-quads, whose rows (4j + e) x ROWB + 16 kq
+  // fall on 16 distinct 4-bank groups for both row strides used (528 and
+  // 144 bytes); the four groups of a wave continue along r, so a load
+  // instruction reads four k-lines in 256-byte runs.
+  static constexpr bool TRANS4 =
+      !KCONT && VEC == 4 && NV == 4 * NTH && ROWS % 16 == 0 && BK % 16 == 0 &&
+      (ROWS / 16) * (BK / 16) * 16 == NTH;
   vec4u reg[PER_THREAD];
 
   // (r, k) of vector v of the tile (r, k are element indices).
@@ -199,6 +210,12 @@ struct TileLoader {
     if (KCONT) {
       r = v / (BK / VEC);
       k = (v % (BK / VEC)) * VEC;
+    } else if (TRANS4) {
+      const int t = v % NTH, i = v / NTH;
+      const int g = t >> 4, jj = t & 3, kk = (t >> 2) & 3;
+      constexpr int GJ = ROWS / 16;  // groups of 4 r-vectors along r
+      r = (4 * (g % GJ) + jj) * 4;
+      k = (4 * (g / GJ) + kk) * 4 + i;
     } else if (BLOCKED) {
       constexpr int NRB = ROWS / VEC / 8;
       const int w = v >> 6, l = v & 63;
@@ -283,6 +300,19 @@ struct TileLoader {
   }
 
   __device__ __forceinline__ void store(char *lds) const {
+    if constexpr (TRANS4) {
+      int r, k;
+      vec_pos(threadIdx.x, r, k);  // reg[i] holds rows r..r+3 at k + i
+      const uint32_t *w0 = reinterpret_cast<const uint32_t *>(&reg[0]);
+      const uint32_t *w1 = reinterpret_cast<const uint32_t *>(&reg[1]);
+      const uint32_t *w2 = reinterpret_cast<const uint32_t *>(&reg[2]);
+      const uint32_t *w3 = reinterpret_cast<const uint32_t *>(&reg[3]);
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        *reinterpret_cast<vec4u *>(lds + (r + e) * ROWB + k * 4) =
+            vec4u{w0[e], w1[e], w2[e], w3[e]};
+      return;
+    }
 #pragma unroll
     for (int i = 0; i < PER_THREAD; ++i) {
       const int v = threadIdx.x + i * NTH;
