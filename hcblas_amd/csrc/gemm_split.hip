@@ -176,24 +176,29 @@ __device__ __forceinline__ void split3(float x, uint16_t &h, uint16_t &m,
   l = f32_to_bf16_rne(r2);
 }
 
-// Packs an R x K operand view (kcont: element (r,k) at X[k + r*ld], else at
-// X[r + k*ld]) into the blocked planes above. A workgroup stages a 64 (r) x
-// 32 (k) tile through LDS with 16-byte reads along the source's contiguous
-// dimension (TileRK); each thread splits 8 consecutive k of one row into the
-// LDS image of the two K blocks' output runs (64 rows x 96 bytes each, one
-// contiguous run per block), which then leave as whole 16-byte slots in
-// order, 1 KiB per wave instruction. (Each thread writing its three 16-byte
-// pieces straight to the planes, 32 of every 96 bytes per instruction, and
-// 4-byte reads: 193 us per 8192^2 operand.)
+// Packs an R x K operand view (KC: element (r,k) at X[k + r*ld], else at
+// X[r + k*ld]) into the blocked planes above. A workgroup stages a tile
+// through LDS with 16-byte reads along the source's contiguous dimension
+// (TileRK): 64 rows x 32 k for a k-contiguous operand, 128 rows x 16 k for an
+// r-contiguous one (512-byte column pieces). Each thread splits 8
+// consecutive k of one row into the LDS image of the tile's output runs (one
+// contiguous run of rows x 96 bytes per K block), which then leave as whole
+// 16-byte slots in order, 1 KiB per wave instruction. (Each thread writing
+// its three 16-byte pieces straight to the planes, 32 of every 96 bytes per
+// instruction, and 4-byte reads: 182-211 us per 8192^2 operand.)
+template <bool KC>
 __global__ __launch_bounds__(256) void split3_pack_kernel(
-    int R, int K, const float *__restrict__ X, int64_t ld, bool kcont,
-    bool vec, uint16_t *__restrict__ P, int nkb) {
-  __shared__ TileRK<float, 64, 32> st;
-  __shared__ vec4u ob[2][64][6];  // [K block][row][16-byte slot]
-  const int r0 = blockIdx.y * 64, k0 = blockIdx.x * 32;
+    int R, int K, const float *__restrict__ X, int64_t ld, bool vec,
+    uint16_t *__restrict__ P, int nkb) {
+  constexpr int RT = KC ? 64 : 128, KT = KC ? 32 : 16;
+  constexpr int GPR = KT / 8;  // threads per row
+  __shared__ TileRK<float, RT, KT> st;
+  __shared__ vec4u ob[KT / 16][RT][6];  // [K block][row][16-byte slot]
+  const int r0 = blockIdx.y * RT, k0 = blockIdx.x * KT;
   const int tid = threadIdx.x;
-  st.load(X, ld, kcont, vec, R, K, r0, k0);
-  const int row = tid >> 2, kg = tid & 3;
+  st.template load_as<KC>(X, ld, vec, R, K, r0, k0);
+  __syncthreads();
+  const int row = tid / GPR, kg = tid % GPR;
   uint32_t hv[4], mv[4], lv[4];
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
@@ -210,17 +215,28 @@ __global__ __launch_bounds__(256) void split3_pack_kernel(
   o[2] = vec4u{mv[0], mv[1], mv[2], mv[3]};
   o[4] = vec4u{lv[0], lv[1], lv[2], lv[3]};
   __syncthreads();
-  const int rows = R - r0 < 64 ? R - r0 : 64;
+  const int rows = R - r0 < RT ? R - r0 : RT;
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
-    const int v = tid + 256 * j;  // 768 slots: 2 blocks x 64 rows x 6
-    const int kbl = v / 384, w = v % 384;
+    const int v = tid + 256 * j;  // KT / 16 blocks x RT rows x 6 = 768
+    const int kbl = v / (RT * 6), w = v % (RT * 6);
     const int kb = (k0 >> 4) + kbl;
     if (w / 6 >= rows || kb >= nkb) continue;
     uint16_t *dst = P + ((int64_t)kb * R + r0) * kS3Row + w * 8;
     __builtin_nontemporal_store(ob[kbl][w / 6][w % 6],
                                 reinterpret_cast<vec4u *>(dst));
   }
+}
+void launch_split3_pack(hipStream_t s, int R, int K, const float *X,
+                        int64_t ld, bool kcont, uint16_t *P, int nkb) {
+  if (kcont)
+    split3_pack_kernel<true>
+        <<<dim3((unsigned)cdiv(nkb * kS3K, 32), (unsigned)cdiv(R, 64)), 256,
+           0, s>>>(R, K, X, ld, vec_aligned(X, ld), P, nkb);
+  else
+    split3_pack_kernel<false>
+        <<<dim3((unsigned)nkb, (unsigned)cdiv(R, 128)), 256, 0, s>>>(
+            R, K, X, ld, vec_aligned(X, ld), P, nkb);
 }
 
 struct Split3Params {
@@ -1429,13 +1445,8 @@ hcblasStatus_t gemm_split_bf16(hcblasHandle_t h, const Canon &c, float alpha,
     const float *A = static_cast<const float *>(bv.A[z]) + c.aOff;
     const float *B = static_cast<const float *>(bv.B[z]) + c.bOff;
     float *Cz = static_cast<float *>(bv.C[z]) + cOff;
-    const unsigned kt = (unsigned)cdiv(nkb * kS3K, 32);
-    dim3 ga(kt, (unsigned)cdiv(c.M, 64));
-    split3_pack_kernel<<<ga, 256, 0, h->stream>>>(
-        c.M, c.K, A, c.lda, c.a_k, vec_aligned(A, c.lda), Ap, nkb);
-    dim3 gb(kt, (unsigned)cdiv(c.N, 64));
-    split3_pack_kernel<<<gb, 256, 0, h->stream>>>(
-        c.N, c.K, B, c.ldb, c.b_k, vec_aligned(B, c.ldb), Bp, nkb);
+    launch_split3_pack(h->stream, c.M, c.K, A, c.lda, c.a_k, Ap, nkb);
+    launch_split3_pack(h->stream, c.N, c.K, B, c.ldb, c.b_k, Bp, nkb);
     st = launch_status();
     if (st != HCBLAS_STATUS_SUCCESS) return st;
     Split3Params p{};
