@@ -472,7 +472,16 @@ __device__ __forceinline__ void gemm_mainloop(
   // LDS latency is not paid between chunks; the larger tiles have no
   // registers to spare for it.
   constexpr bool PIPE = WM * WN == 8 && FM * FN <= 2;
-  auto compute = [&](const char *cur) {
+  // The large 8-wave tiles (256 x 256 and 256 x 128, two waves per SIMD)
+  // issue the step's LDS writes and global loads among the first chunk's
+  // MFMAs, behind that chunk's fragment reads, instead of before them: both
+  // waves of a SIMD leave the barrier together, and the writes would
+  // otherwise hold the MFMA pipe idle while they drain and the fragment
+  // reads queue behind them (8192^3 SGEMM 3-7% faster over the four
+  // transpose forms, DGEMM 6-11%, 3072^3 SGEMM on 256 x 128 tiles 10%).
+  constexpr bool ILV = !PIPE && WM * WN == 8 && FM * FN >= 8;
+  // hook: the step's LDS writes and loads, run by ILV tiles inside chunk 0.
+  auto compute = [&](const char *cur, auto hook) {
     const char *As = cur;
     const char *Bs = cur + A_BYTES;
     vec4u afp[PIPE ? 2 : 1][FM], bfp[PIPE ? 2 : 1][FN];
@@ -495,6 +504,9 @@ __device__ __forceinline__ void gemm_mainloop(
         if (c + 1 < BK / CK) frag(c + 1, (c + 1) & 1);
       } else {
         frag(c, 0);
+      }
+      if constexpr (ILV) {
+        if (c == 0) hook();
       }
       const vec4u(&af)[FM] = afp[set];
       const vec4u(&bf)[FN] = bfp[set];
@@ -532,8 +544,25 @@ __device__ __forceinline__ void gemm_mainloop(
         }
         __builtin_amdgcn_sched_barrier(0);
       }
+      if constexpr (ILV) {
+        if (c == 0) {
+          constexpr int NMF = FM * FN * Tr::STEPS, NRD = FM + FN;
+          __builtin_amdgcn_sched_group_barrier(0x100, NRD, 0);  // DS read
+          // One LDS write, then one global load, every 4 MFMAs.
+#pragma unroll
+          for (int i = 0; i < NMF; ++i) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+            if (i % 4 == 0)
+              __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);  // DS write
+            if (i % 4 == 2)
+              __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // VMEM read
+          }
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      }
     }
   };
+  auto none = [] {};
   auto buf = [&](int kt) { return lds + (kt & 1) * (A_BYTES + B_BYTES); };
 
   if (nk == 0) return;
@@ -555,16 +584,23 @@ __device__ __forceinline__ void gemm_mainloop(
   __syncthreads();
   int kt = 0;
   for (; kt + 2 < nfull; ++kt) {
-    store(kt + 1);
-    load(kt + 2, true);
-    __builtin_amdgcn_sched_barrier(0);
-    compute(buf(kt));
+    auto step = [&] {
+      store(kt + 1);
+      load(kt + 2, true);
+    };
+    if constexpr (ILV) {
+      compute(buf(kt), step);
+    } else {
+      step();
+      __builtin_amdgcn_sched_barrier(0);
+      compute(buf(kt), none);
+    }
     __syncthreads();
   }
   for (; kt < nk; ++kt) {
     if (kt + 1 < nk) store(kt + 1);
     if (kt + 2 < nk) load(kt + 2, kt + 2 < nfull);
-    compute(buf(kt));
+    compute(buf(kt), none);
     __syncthreads();
   }
 }

@@ -73,10 +73,13 @@ hcblasStatus_t launch_tiles(hipStream_t stream, GemmParams p, bool a_k,
 
 // Macro-tile choice (measured at 8192^3 on MI355X, the largest tile whose
 // grid still covers every CU is taken):
-//   FP32  256x256x32, 8 waves of 128x64 on 32x32 MFMA blocks (134 TF/s,
-//         85% of peak), then 256x128 8 waves (128), 128x128 4 waves, 64x64;
-//   FP64  256x128x16, 8 waves of 64x64 (66.9 TF/s NN / 64.5 TN / 66.9 NT /
-//         66.1 TT, 82-85%; 128x256 2x4 is as fast on NN, 128x128 and
+//   FP32  256x256x32, 8 waves of 128x64 on 32x32 MFMA blocks (146-148
+//         TF/s, 93-94% of peak, with the step's LDS writes and loads issued
+//         among the first chunk's MFMAs), then 256x128 8 waves, 128x128 4
+//         waves, 64x64;
+//   FP64  256x128x16, 8 waves of 64x64 (71.6 TF/s NN and TN, 91%, with the
+//         same interleave; 66.9 / 64.5 before it; 128x256 2x4 was as fast
+//         on NN, 128x128 and
 //         smaller tiles stay at 75% or below; aligned operands only, the
 //         element-wise loaders spill at this size), then 128x64x16 8 waves
 //         (58.5), then 64x64 4 waves;
