@@ -101,6 +101,7 @@ typedef int intx16 __attribute__((ext_vector_type(16)));
 // 16-byte fragment / staging register as a native vector (the HIP uint4
 // struct defeats scalar replacement of the staging arrays).
 typedef unsigned int vec4u __attribute__((ext_vector_type(4)));
+typedef unsigned int vec2u __attribute__((ext_vector_type(2)));
 
 template <Elem E, int MI>
 struct AccOf;
@@ -203,6 +204,24 @@ struct TileLoader {
   static constexpr bool TRANS4 =
       !KCONT && VEC == 4 && NV == 4 * NTH && ROWS % 16 == 0 && BK % 16 == 0 &&
       (ROWS / 16) * (BK / 16) * 16 == NTH;
+  // 16-bit transposing tiles with four vectors per thread and 64-deep K
+  // steps: each thread loads the four k-lines of one 8-row vector and writes
+  // its 8 rows x 4 k with eight 8-byte LDS stores along k (16-bit selects
+  // pair the k values) instead of thirty-two 2-byte ones. The 32 lanes of a
+  // half-wave take 2 consecutive r-vectors x the 16 k-quads, whose slots
+  // (8 jr + e) x 144 + 8 kq are 32 distinct 8-byte slots of 256 bytes.
+  static constexpr bool TRANS4H = !KCONT && VEC == 8 && BK == 64 &&
+                                  NV == 4 * NTH && (ROWS / 8) * 16 == NTH;
+  // The same with two vectors per thread (64-row tiles of 4 waves): a
+  // thread loads two k-lines of one r-vector and writes its rows with
+  // 8-byte (FP32, 32-deep steps) or 4-byte (16-bit, 64-deep steps) stores
+  // along k. FP32: wave w takes r-vectors 4w..4w+3, its half-waves r-vectors
+  // {0, 2} and {1, 3} of those x the 16 k-pairs (row offsets 0 / 128 and
+  // 64 / 192 of 256 bytes); 16-bit: 2 r-vectors x 32 k-pairs per wave.
+  static constexpr bool TRANS2 = !KCONT && VEC == 4 && BK == 32 &&
+                                 NV == 2 * NTH && ROWS / 4 == NTH / 16;
+  static constexpr bool TRANS2H = !KCONT && VEC == 8 && BK == 64 &&
+                                  NV == 2 * NTH && (ROWS / 8) * 32 == NTH;
   vec4u reg[PER_THREAD];
 
   // (r, k) of vector v of the tile (r, k are element indices).
@@ -216,6 +235,18 @@ struct TileLoader {
       constexpr int GJ = ROWS / 16;  // groups of 4 r-vectors along r
       r = (4 * (g % GJ) + jj) * 4;
       k = (4 * (g / GJ) + kk) * 4 + i;
+    } else if (TRANS2) {
+      const int t = v % NTH, i = v / NTH, l = t & 63;
+      r = (4 * (t >> 6) + ((l >> 4) & 1) * 2 + (l >> 5)) * 4;
+      k = (l & 15) * 2 + i;
+    } else if (TRANS2H) {
+      const int t = v % NTH, i = v / NTH;
+      r = (t >> 5) * 8;
+      k = (t & 31) * 2 + i;
+    } else if (TRANS4H) {
+      const int t = v % NTH, i = v / NTH;
+      r = (t >> 4) * 8;
+      k = (t & 15) * 4 + i;
     } else if (BLOCKED) {
       constexpr int NRB = ROWS / VEC / 8;
       const int w = v >> 6, l = v & 63;
@@ -303,14 +334,49 @@ struct TileLoader {
     if constexpr (TRANS4) {
       int r, k;
       vec_pos(threadIdx.x, r, k);  // reg[i] holds rows r..r+3 at k + i
-      const uint32_t *w0 = reinterpret_cast<const uint32_t *>(&reg[0]);
-      const uint32_t *w1 = reinterpret_cast<const uint32_t *>(&reg[1]);
-      const uint32_t *w2 = reinterpret_cast<const uint32_t *>(&reg[2]);
-      const uint32_t *w3 = reinterpret_cast<const uint32_t *>(&reg[3]);
 #pragma unroll
       for (int e = 0; e < 4; ++e)
         *reinterpret_cast<vec4u *>(lds + (r + e) * ROWB + k * 4) =
-            vec4u{w0[e], w1[e], w2[e], w3[e]};
+            vec4u{reg[0][e], reg[1][e], reg[2][e], reg[3][e]};
+      return;
+    }
+    if constexpr (TRANS2) {
+      int r, k;
+      vec_pos(threadIdx.x, r, k);  // reg[i] holds rows r..r+3 at k + i
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        *reinterpret_cast<vec2u *>(lds + (r + e) * ROWB + k * 4) =
+            vec2u{reg[0][e], reg[1][e]};
+      return;
+    }
+    if constexpr (TRANS2H) {
+      int r, k;
+      vec_pos(threadIdx.x, r, k);  // reg[i] holds rows r..r+7 at k + i
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int d = e >> 1;
+        *reinterpret_cast<uint32_t *>(lds + (r + e) * ROWB + k * 2) =
+            (e & 1) ? (reg[0][d] >> 16) | (reg[1][d] & 0xffff0000u)
+                    : (reg[0][d] & 0xffffu) | (reg[1][d] << 16);
+      }
+      return;
+    }
+    if constexpr (TRANS4H) {
+      int r, k;
+      vec_pos(threadIdx.x, r, k);  // reg[i] holds rows r..r+7 at k + i
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        // Row r + e is the low (e even) or high half of dword e / 2.
+        const int d = e >> 1;
+        const uint32_t lo =
+            (e & 1) ? (reg[0][d] >> 16) | (reg[1][d] & 0xffff0000u)
+                    : (reg[0][d] & 0xffffu) | (reg[1][d] << 16);
+        const uint32_t hi =
+            (e & 1) ? (reg[2][d] >> 16) | (reg[3][d] & 0xffff0000u)
+                    : (reg[2][d] & 0xffffu) | (reg[3][d] << 16);
+        *reinterpret_cast<vec2u *>(lds + (r + e) * ROWB + k * 2) =
+            vec2u{lo, hi};
+      }
       return;
     }
 #pragma unroll
@@ -480,7 +546,12 @@ __device__ __forceinline__ void gemm_mainloop(
   // reads queue behind them (8192^3 SGEMM 3-7% faster over the four
   // transpose forms, DGEMM 6-11%, 3072^3 SGEMM on 256 x 128 tiles 10%).
   constexpr bool ILV = !PIPE && WM * WN == 8 && FM * FN >= 8;
-  // hook: the step's LDS writes and loads, run by ILV tiles inside chunk 0.
+  // 16-bit tiles have 8 MFMAs per chunk, too few to carry a whole step's
+  // writes and loads: their four chunks take A's LDS writes, B's, A's loads
+  // and B's loads in turn (8192^3 HGEMM 2-4% faster than all in chunk 0).
+  constexpr bool SPREAD = ILV && BK / CK == 4 && FM * FN * Tr::STEPS < 16;
+  // hook(c): the step's LDS writes and loads, or the part of them that
+  // chunk c carries.
   auto compute = [&](const char *cur, auto hook) {
     const char *As = cur;
     const char *Bs = cur + A_BYTES;
@@ -506,7 +577,7 @@ __device__ __forceinline__ void gemm_mainloop(
         frag(c, 0);
       }
       if constexpr (ILV) {
-        if (c == 0) hook();
+        if (SPREAD || c == 0) hook(c);
       }
       const vec4u(&af)[FM] = afp[set];
       const vec4u(&bf)[FN] = bfp[set];
@@ -544,7 +615,19 @@ __device__ __forceinline__ void gemm_mainloop(
         }
         __builtin_amdgcn_sched_barrier(0);
       }
-      if constexpr (ILV) {
+      if constexpr (SPREAD) {
+        constexpr int NMF = FM * FN * Tr::STEPS, NRD = FM + FN;
+        __builtin_amdgcn_sched_group_barrier(0x100, NRD, 0);  // DS read
+#pragma unroll
+        for (int i = 0; i < NMF; ++i) {
+          __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+          if (c < 2)
+            __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);  // DS write
+          else
+            __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // VMEM read
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      } else if constexpr (ILV) {
         if (c == 0) {
           constexpr int NMF = FM * FN * Tr::STEPS, NRD = FM + FN;
           __builtin_amdgcn_sched_group_barrier(0x100, NRD, 0);  // DS read
@@ -562,7 +645,7 @@ __device__ __forceinline__ void gemm_mainloop(
       }
     }
   };
-  auto none = [] {};
+  auto none = [](int) {};
   auto buf = [&](int kt) { return lds + (kt & 1) * (A_BYTES + B_BYTES); };
 
   if (nk == 0) return;
@@ -584,14 +667,21 @@ __device__ __forceinline__ void gemm_mainloop(
   __syncthreads();
   int kt = 0;
   for (; kt + 2 < nfull; ++kt) {
-    auto step = [&] {
-      store(kt + 1);
-      load(kt + 2, true);
+    auto step = [&](int c) {
+      if constexpr (SPREAD) {
+        if (c == 0) la.store(buf(kt + 1));
+        if (c == 1) lb.store(buf(kt + 1) + A_BYTES);
+        if (c == 2) la.load_step(A, lda, m0, (kt + 2) * BK, M, K, true);
+        if (c == 3) lb.load_step(B, ldb, n0, (kt + 2) * BK, N, K, true);
+      } else {
+        store(kt + 1);
+        load(kt + 2, true);
+      }
     };
     if constexpr (ILV) {
       compute(buf(kt), step);
     } else {
-      step();
+      step(0);
       __builtin_amdgcn_sched_barrier(0);
       compute(buf(kt), none);
     }
