@@ -35,7 +35,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6, "bf16": 2516.6, "int8": 5033.0}
+PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6, "bf16": 2516.6, "fp16": 2516.6,
+               "int8": 5033.0}
 
 
 def parse():
@@ -59,6 +60,7 @@ CASES = {
     "sgemm_8192": ("s", 8192, "hcblasSgemm", False),
     "dgemm_8192": ("d", 8192, "hcblasDgemm", False),
     "sgemm_rowmajor_1024": ("s", 1024, "hcblas_sgemm", True),
+    "hgemm_8192": ("h", 8192, "hcblas_hgemm", False),
     "sgemm_split_bf16_8192": ("s", 8192, "hcblas_sgemm_split_bf16", False),
     "sgemm_ozaki_crt_8192": ("s", 8192, "hcblas_sgemm_ozaki_crt", False),
     "dgemm_ozaki_crt_8192": ("d", 8192, "hcblas_dgemm_ozaki_crt", False),
@@ -94,9 +96,9 @@ def run_case(hc, h, torch, name, reps):
             (torch_ext.gemm_batched(A, B) * W).sum().backward()
             return 0
     else:
-        dt = torch.float32 if tag == "s" else torch.float64
-        A = torch.rand(n * n, generator=g, device="cuda", dtype=dt) * 2 - 1
-        B = torch.rand(n * n, generator=g, device="cuda", dtype=dt) * 2 - 1
+        dt = {"s": torch.float32, "d": torch.float64, "h": torch.float16}[tag]
+        A = (torch.rand(n * n, generator=g, device="cuda") * 2 - 1).to(dt)
+        B = (torch.rand(n * n, generator=g, device="cuda") * 2 - 1).to(dt)
         C = torch.zeros(n * n, device="cuda", dtype=dt)
         f = getattr(api, entry)
         if entry.startswith("hcblas_"):
@@ -417,6 +419,38 @@ def gemm_section(hc, h, torch, oracle, n, reps=20, nc=4096):
     out["sgemm_rowmajor_1024"]["torch_matmul_ms"] = round(tms, 4)
     out["sgemm_rowmajor_1024"]["torch_matmul_peak_frac"] = round(
         2.0 * m ** 3 / (tms * 1e-3) / 1e12 / PEAK_TFLOPS["fp32"], 4)
+    del A, B, C
+    # HGEMM (binary16 in / out, FP32 accumulation) at 8192^3, column-major
+    # NN and TN: median of 5 single calls, with torch's half matmul on the
+    # same NN operands for scale.
+    n = 8192
+    Ah = (torch.rand(n * n, generator=g, device="cuda") * 2 - 1).half()
+    Bh = (torch.rand(n * n, generator=g, device="cuda") * 2 - 1).half()
+    Ch = torch.zeros(n * n, device="cuda").half()
+    one = 1.0
+
+    def med(f):
+        f()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(5):
+            s.record()
+            f()
+            e.record()
+            torch.cuda.synchronize()
+            ts.append(s.elapsed_time(e))
+        return sorted(ts)[2]
+    hg = {}
+    for ta in (0, 1):
+        ms = med(lambda: api.hcblas_hgemm(h, 1, ta, 0, n, n, n, one, Ah, n,
+                                          Bh, n, 0.0, Ch, n, 0, 0, 0))
+        tf = 2.0 * n ** 3 / (ms * 1e-3) / 1e12
+        hg["nt"[ta] + "n"] = {"ms": round(ms, 4), "tflops": round(tf, 1),
+                              "peak_frac": round(tf / PEAK_TFLOPS["fp16"], 4)}
+    Am, Bm = Ah.view(n, n), Bh.view(n, n)
+    tms = med(lambda: torch.mm(Bm, Am))
+    hg["torch_matmul_ms"] = round(tms, 4)
+    out["hgemm_8192"] = hg
     return out
 
 
