@@ -520,26 +520,34 @@ def test_large_tiles_with_beta(hc, handle, orc, torch_cuda, p, M, N, K):
                            seed=M + 7 * N + K + 23 * ta + 29 * tb)
 
 
-def test_hgemm_large_tiles_with_beta(hc, handle, torch_cuda):
-    """HGEMM on 256x256 tiles (FP16 in and out, FP32 accumulate) with
-    beta != 0: sampled entries against the FP64 value rounded to half."""
+@pytest.mark.parametrize("M,N,K", [(4096, 4096, 512), (2048, 2048, 328)])
+@pytest.mark.parametrize("ta,tb", [(N_, N_), (N_, T_), (T_, N_), (T_, T_)])
+def test_hgemm_large_tiles_with_beta(hc, handle, torch_cuda, M, N, K, ta, tb):
+    """HGEMM on 256x256 (4096^2) and 128x128 (2048^2, K with a partial last
+    step) tiles, FP16 in and out
+    with FP32 accumulation, every transpose form (the r-contiguous operands
+    are transposed in registers on their way to LDS), beta != 0: sampled
+    entries against the FP64 value rounded to half."""
     torch = torch_cuda
     api = hc.api
-    M, N, K = 4096, 4096, 512
-    g = torch.Generator(device="cuda").manual_seed(12)
+    g = torch.Generator(device="cuda").manual_seed(12 + 2 * ta + tb)
     A = (torch.rand(M * K, generator=g, device="cuda") * 2 - 1).half()
     B = (torch.rand(K * N, generator=g, device="cuda") * 2 - 1).half()
     C0 = (torch.rand(M * N, generator=g, device="cuda") * 2 - 1).half()
     C = C0.clone()
-    st = api.hcblas_hgemm(handle, COL, N_, N_, M, N, K, 0.75, A.data_ptr(), M,
-                          B.data_ptr(), K, 0.5, C.data_ptr(), M, 0, 0, 0)
+    lda = M if ta == N_ else K
+    ldb = K if tb == N_ else N
+    st = api.hcblas_hgemm(handle, COL, ta, tb, M, N, K, 0.75, A.data_ptr(),
+                          lda, B.data_ptr(), ldb, 0.5, C.data_ptr(), M, 0, 0,
+                          0)
     assert st == 0
     torch.cuda.synchronize()
     rng = np.random.default_rng(3)
     rows = torch.as_tensor(rng.integers(0, M, 1024), device="cuda")
     cols = torch.as_tensor(rng.integers(0, N, 1024), device="cuda")
-    Am = A.view(K, M).t().double()     # A(m, k)
-    Bm = B.view(N, K).t().double()     # B(k, n)
+    # op(A)(m, k) and op(B)(k, n) as dense FP64 views of the stored arrays
+    Am = (A.view(K, M).t() if ta == N_ else A.view(M, K)).double()
+    Bm = (B.view(N, K).t() if tb == N_ else B.view(K, N)).double()
     ref = 0.75 * (Am[rows] * Bm[:, cols].t()).sum(1) + \
         0.5 * C0.view(N, M).t().double()[rows, cols]
     got = C.view(N, M).t()[rows, cols].double().cpu().numpy()
