@@ -545,7 +545,10 @@ __device__ __forceinline__ void gemm_mainloop(
   // otherwise hold the MFMA pipe idle while they drain and the fragment
   // reads queue behind them (8192^3 SGEMM 3-7% faster over the four
   // transpose forms, DGEMM 6-11%, 3072^3 SGEMM on 256 x 128 tiles 10%).
-  constexpr bool ILV = !PIPE && WM * WN == 8 && FM * FN >= 8;
+  // The 4-wave 128 x 128 FP32 tile (64 x 64 per wave on 16 x 16 blocks, two
+  // workgroups per CU) gains the same way: 2048^3 SGEMM 166 against 187 us.
+  constexpr bool ILV = !PIPE && FM * FN >= 8 &&
+                       (WM * WN == 8 || (WM * WN == 4 && MI == 16));
   // 16-bit tiles have 8 MFMAs per chunk, too few to carry a whole step's
   // writes and loads: their four chunks take A's LDS writes, B's, A's loads
   // and B's loads in turn (8192^3 HGEMM 2-4% faster than all in chunk 0).
