@@ -546,9 +546,12 @@ __device__ __forceinline__ void gemm_mainloop(
   // reads queue behind them (8192^3 SGEMM 3-7% faster over the four
   // transpose forms, DGEMM 6-11%, 3072^3 SGEMM on 256 x 128 tiles 10%).
   // The 4-wave 128 x 128 FP32 tile (64 x 64 per wave on 16 x 16 blocks, two
-  // workgroups per CU) gains the same way: 2048^3 SGEMM 166 against 187 us.
-  constexpr bool ILV = !PIPE && FM * FN >= 8 &&
-                       (WM * WN == 8 || (WM * WN == 4 && MI == 16));
+  // workgroups per CU) gains the same way (2048^3 SGEMM 166 against 187 us),
+  // and so does the 8-wave 128 x 64 FP64 tile (2048^3 DGEMM 294 against
+  // 344 us); the 4-wave 128 x 128 16-bit tile, with 4 MFMAs per chunk, was
+  // 5-18% slower with it.
+  constexpr bool ILV = !PIPE && ((WM * WN == 8 && FM * FN >= 4) ||
+                                 (WM * WN == 4 && MI == 16 && FM * FN >= 8));
   // 16-bit tiles have 8 MFMAs per chunk, too few to carry a whole step's
   // writes and loads: their four chunks take A's LDS writes, B's, A's loads
   // and B's loads in turn (8192^3 HGEMM 2-4% faster than all in chunk 0).
@@ -634,13 +637,16 @@ __device__ __forceinline__ void gemm_mainloop(
         if (c == 0) {
           constexpr int NMF = FM * FN * Tr::STEPS, NRD = FM + FN;
           __builtin_amdgcn_sched_group_barrier(0x100, NRD, 0);  // DS read
-          // One LDS write, then one global load, every 4 MFMAs.
+          // One LDS write, then one global load, every SP MFMAs (4, or
+          // fewer where the chunk is short).
+          constexpr int NWR = LA::PER_THREAD + LB::PER_THREAD;
+          constexpr int SP = NMF >= 4 * NWR ? 4 : (NMF / NWR >= 2 ? 2 : 1);
 #pragma unroll
           for (int i = 0; i < NMF; ++i) {
             __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
-            if (i % 4 == 0)
+            if (i % SP == 0)
               __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);  // DS write
-            if (i % 4 == 2)
+            if (i % SP == SP / 2)
               __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // VMEM read
           }
           __builtin_amdgcn_sched_barrier(0);
