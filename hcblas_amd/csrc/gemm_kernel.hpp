@@ -533,11 +533,14 @@ __device__ __forceinline__ void gemm_mainloop(
   const int nfull = KFULL ? nk : K / BK;
 
   // MFMAs over one LDS buffer. Small wave tiles (the 64 x 64 tiles of 8
-  // waves, at most 2 fragments of each operand) read chunk c + 1's
-  // fragments into a second register set before chunk c's MFMAs, so the
-  // LDS latency is not paid between chunks; the larger tiles have no
-  // registers to spare for it.
-  constexpr bool PIPE = WM * WN == 8 && FM * FN <= 2;
+  // waves, at most 2 fragments of each operand, and the 4-wave 64 x 64 FP64
+  // tiles, 2 x 2 fragments) read chunk c + 1's fragments into a second
+  // register set before chunk c's MFMAs, so the LDS latency is not paid
+  // between chunks (1024^3 DGEMM 71.5 against 75.3 us); the larger tiles
+  // have no registers to spare for it, and the 4-wave 128 x 128 16-bit tile
+  // measured no faster with it.
+  constexpr bool PIPE = (WM * WN == 8 && FM * FN <= 2) ||
+                        (E == Elem::F64 && WM * WN == 4 && FM * FN == 4);
   // The large 8-wave tiles (256 x 256 and 256 x 128, two waves per SIMD)
   // issue the step's LDS writes and global loads among the first chunk's
   // MFMAs, behind that chunk's fragment reads, instead of before them: both
