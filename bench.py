@@ -61,6 +61,9 @@ CASES = {
     "dgemm_8192": ("d", 8192, "hcblasDgemm", False),
     "sgemm_rowmajor_1024": ("s", 1024, "hcblas_sgemm", True),
     "hgemm_8192": ("h", 8192, "hcblas_hgemm", False),
+    "sgemm_2048": ("s", 2048, "hcblasSgemm", False),
+    "dgemm_2048": ("d", 2048, "hcblasDgemm", False),
+    "dgemm_1024": ("d", 1024, "hcblasDgemm", False),
     "sgemm_split_bf16_8192": ("s", 8192, "hcblas_sgemm_split_bf16", False),
     "sgemm_ozaki_crt_8192": ("s", 8192, "hcblas_sgemm_ozaki_crt", False),
     "dgemm_ozaki_crt_8192": ("d", 8192, "hcblas_dgemm_ozaki_crt", False),
