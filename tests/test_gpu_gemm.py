@@ -511,8 +511,12 @@ def test_bf16_split_k_single_problem(hc, handle, torch_cuda):
 
 
 # Shapes large enough for the biggest tiles (256x256 FP32, 256x128 FP64),
-# with beta != 0, so the LDS-staged epilogue of those tiles reads C.
-@pytest.mark.parametrize("p,M,N,K", [("s", 4096, 4096, 256), ("d", 4096, 2048, 64)])
+# and 2048^2 shapes for the next ones (128x128 FP32 on 4 waves, 128x64 FP64
+# on 8 waves; K = 200 ends in a partial K step), all of which issue a
+# step's LDS writes and loads among its MFMAs; beta != 0, so the
+# LDS-staged epilogue reads C.
+@pytest.mark.parametrize("p,M,N,K", [("s", 4096, 4096, 256), ("d", 4096, 2048, 64),
+                                     ("s", 2048, 2048, 200), ("d", 2048, 2048, 200)])
 def test_large_tiles_with_beta(hc, handle, orc, torch_cuda, p, M, N, K):
     for ta in (N_, T_):
         for tb in (N_, T_):
