@@ -82,7 +82,7 @@ hcblasStatus_t launch_tiles(hipStream_t stream, GemmParams p, bool a_k,
 //         on NN, 128x128 and
 //         smaller tiles stay at 75% or below; aligned operands only, the
 //         element-wise loaders spill at this size), then 128x64x16 8 waves
-//         (58.5), then 64x64 4 waves;
+//         (58.5), then 64x64 8 waves;
 //   F16 / BF16 256x256x64, 8 waves on 32x32 blocks, then 128x128 8 waves,
 //         then 64x64.
 template <Elem E, typename OutT>
@@ -124,11 +124,11 @@ hcblasStatus_t launch_gemm(hipStream_t stream, int num_cus, GemmParams p,
       return launch_tiles<E, OutT, 128, 128, TileCfg<E>::BK_BIG, 2, 4, 32>(
           stream, p, a_k, b_k, vload, batch);
   }
-  // Small problems: 64x64 tiles, 8 waves of 32x16 for the 32-bit and
-  // 16-bit types (more waves per CU to cover the load latency when there
-  // are only about as many tiles as CUs: 1024^3 SGEMM 0.0376 ms against
-  // 0.0382 with 4 waves, the conv-net shape set 24.6 TF/s against 23.8),
-  // 4 waves of 32x32 for FP64. FP32 with K >= 512 takes 128-wide K steps:
+  // Small problems: 64x64 tiles, 8 waves of 32x16 for every type (more
+  // waves per CU to cover the load latency when there are only about as
+  // many tiles as CUs: 1024^3 SGEMM 0.0376 ms against 0.0382 with 4 waves,
+  // the conv-net shape set 24.6 TF/s against 23.8; 1024^3 DGEMM 58.8 us
+  // against 71.6). FP32 with K >= 512 takes 128-wide K steps:
   // a 32-wide step is 16 MFMAs per wave, too short to cover the global loads
   // issued one step ahead (1024^3 kernel 27.5 us against 31.6 with BK = 32,
   // 29.5 with 64); shorter K keeps 32 so the last step pads little.
@@ -137,10 +137,7 @@ hcblasStatus_t launch_gemm(hipStream_t stream, int num_cus, GemmParams p,
       return launch_tiles<E, OutT, 64, 64, 128, 2, 4>(stream, p, a_k, b_k,
                                                       vload, batch);
   }
-  if constexpr (E != Elem::F64)
-    return launch_tiles<E, OutT, 64, 64, TileCfg<E>::BK_SMALL, 2, 4>(
-        stream, p, a_k, b_k, vload, batch);
-  return launch_tiles<E, OutT, 64, 64, TileCfg<E>::BK_SMALL, 2, 2>(
+  return launch_tiles<E, OutT, 64, 64, TileCfg<E>::BK_SMALL, 2, 4>(
       stream, p, a_k, b_k, vload, batch);
 }
 
