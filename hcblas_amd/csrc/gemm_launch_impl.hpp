@@ -147,7 +147,9 @@ hcblasStatus_t launch_gemm_splitk(hipStream_t stream, int num_cus,
                                   int ks, typename ElemT<E>::Acc *W) {
   using Acc = typename ElemT<E>::Acc;
   constexpr int BM = 64, BN = 64, BK = TileCfg<E>::BK_SMALL;
-  constexpr int WM = 2, WN = E == Elem::F64 ? 2 : 4;
+  // 8 waves of 32 x 16 for every type (FP64 256 x 128 x 16384 96 us
+  // against 123 with 4 waves of 32 x 32).
+  constexpr int WM = 2, WN = 4;
   p.tiles_m = (int)cdiv(p.M, BM);
   p.tiles_n = (int)cdiv(p.N, BN);
   dim3 grid((unsigned)(p.tiles_m * p.tiles_n), 1, (unsigned)S);
